@@ -1,0 +1,67 @@
+// Device pointer bundle and kernel launchers of the AFAR stage pipeline.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+namespace afar {
+
+template <typename R> struct VT;
+template <> struct VT<double> {
+    using v4 = double4;
+    using v2 = double2;
+};
+template <> struct VT<float> {
+    using v4 = float4;
+    using v2 = float2;
+};
+
+// Everything a kernel reads or writes, passed by value.  Cell arrays hold nc
+// real cells followed (where noted) by nb boundary pseudo-cells.
+template <typename R>
+struct DevPtrs {
+    int nc, nf, nb, maxS, maxC, ntiles;
+    typename VT<R>::v4* W;     // nc+nb primitive p, Ux, Uy, T
+    double4* U;                // nc conserved rho, rhoUx, rhoUy, rhoE
+    double4* Uold;             // nc
+    const double* vol;         // nc
+    typename VT<R>::v2 *gp, *gu, *gv, *gT;  // nc: grad p; (dUx/dx, dUx/dy); (dUy/dx, dUy/dy); grad T
+    typename VT<R>::v4* lim;   // nc limiters p, Ux, Uy, T
+    int8_t* smooth;            // nc+nb smoothness flag (shockIndicator[2c])
+    int8_t* ftype;             // nc final reconstruction type (shockIndicator[2c+1])
+    const uint8_t* scnt;       // nc stencil size
+    const uint8_t* ccnt;       // nc compact stencil size
+    const uint16_t* sten;      // sliced ELL, LDS-local stencil ids
+    const int32_t* slot_face;  // sliced ELL, +-(face+1)
+    const double2* slot_d;     // sliced ELL, face centre - cell centre
+    const double2* slot_e;     // sliced ELL, compact neighbour - cell centre
+    typename VT<R>::v2* lsq;   // sliced ELL, least-squares weights
+    const R* rbf;              // sliced ELL, RBF coefficients (face slot, stencil member)
+    const int32_t* halo_ptr;
+    const int32_t* halo;
+    const int2* fcell;         // nf own, nei
+    const double4* fgeo;       // 2*nf: (Sfx, Sfy, magSf, deltaR), (dxo, dyo, dxn, dyn)
+    typename VT<R>::v4* fv;    // 2*nf reconstructed face values, [2f + side]
+    double4* flux;             // nf + nb
+    const int32_t* bcell;      // nb
+    const uint8_t* blabel;     // nb
+    const double4* bgeo;       // nb (Sfx, Sfy, magSf, 0)
+    const double4* bval;       // nb fixed values of label 6
+    double* dt;                // device time step
+    unsigned long long* dt_accum;
+    unsigned int* dt_count;
+};
+
+template <typename R> void launch_lsq(const DevPtrs<R>& p, hipStream_t s);
+template <typename R> void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s);
+template <typename R> void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s);
+template <typename R> void launch_bvd(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s);
+template <typename R> void launch_flux(const DevPtrs<R>& p, double Rg, double Cv, hipStream_t s);
+template <typename R>
+void launch_update(const DevPtrs<R>& p, double Rg, double gamma, double dt, const double* dt_ptr, double b1, double b2,
+                   double b3, int last, hipStream_t s);
+template <typename R> hipError_t set_lds_limits(size_t lds);
+void launch_math_selftest(const double* x, double* out, int n, hipStream_t s);
+
+}  // namespace afar

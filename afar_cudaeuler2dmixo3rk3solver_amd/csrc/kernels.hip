@@ -1,0 +1,639 @@
+// gfx950 kernels of the AFAR SSP-RK3 stage pipeline.
+//
+// Per RK stage: reconstruct (cell) -> bvd (cell) -> flux (face) -> update
+// (cell: residual gather + RK update + boundary ghosts); one CFL pass per
+// step.  The kernels are templated on the reconstruction/flux type R: R =
+// double is the all-FP64 transcription of the reference kernels, R = float is
+// the mixed-precision path (state, residual and RK update stay FP64).
+//
+// Reference kernels: src/CUDA/cudaEuler2DmixO3RK3/cudaLimiter.h:35-464,
+// cudaRiemannSolver.h:7-271, cudaFieldOperation.h:51-268,
+// src/CUDA/BCs/2D/cudaEulerBCs.h:42-73, launch order GPUevolve.cu:207-306.
+//
+// Arithmetic is written operation-for-operation in the reference's order and
+// the file is built with -ffp-contract=off, so the FP64 path reproduces the
+// host oracle bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "afar_math.h"
+#include "kernels.h"
+
+namespace afar {
+
+// Thresholds of the reference (cudaLimiter.h:4-7), in the working type.
+template <typename R> struct K {
+    static constexpr R SMALL = (R)1e-5;
+    static constexpr R VSMALL = (R)1e-10;
+    static constexpr R BVD_FACTOR = (R)0.5;
+    static constexpr R VALID_LIM_FACTOR = (R)0.9;
+};
+
+__device__ __forceinline__ size_t sell(int c, int k, int Kw) {
+    return ((size_t)(c >> 6) * Kw + k) * 64 + (c & 63);
+}
+
+// Consecutive tiles go to the same XCD (blocks are dispatched round-robin over
+// the 8 XCDs), so a tile's halo was usually just read through the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblocks) {
+    const int per = nblocks >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+template <typename R> __device__ __forceinline__ R kexp(R x);
+template <> __device__ __forceinline__ double kexp<double>(double x) { return afar_exp(x); }
+template <> __device__ __forceinline__ float kexp<float>(float x) { return expf(x); }
+template <typename R> __device__ __forceinline__ R kerfc(R x);
+template <> __device__ __forceinline__ double kerfc<double>(double x) { return afar_erfc(x); }
+template <> __device__ __forceinline__ float kerfc<float>(float x) { return erfcf(x); }
+
+// ---------------------------------------------------------------- K3 least squares
+
+template <typename R>
+__global__ void k_lsq(DevPtrs<R> p) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= p.nc) return;
+    const int C = p.ccnt[c];
+    double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
+    for (int i = 0; i < C; ++i) {
+        const double2 e = p.slot_e[sell(c, i, p.maxC)];
+        Sxx += e.x * e.x;
+        Sxy += e.x * e.y;
+        Syy += e.y * e.y;
+    }
+    for (int i = 0; i < C; ++i) {
+        const double2 e = p.slot_e[sell(c, i, p.maxC)];
+        typename VT<R>::v2 m;
+        m.x = (R)((Syy * e.x - Sxy * e.y) / (Sxx * Syy - Sxy * Sxy));
+        m.y = (R)((-Sxy * e.x + Sxx * e.y) / (Sxx * Syy - Sxy * Sxy));
+        p.lsq[sell(c, i, p.maxC)] = m;
+    }
+}
+
+// ---------------------------------------------------------------- K4 time step
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_cfl(DevPtrs<R> p, double R_, double Cv, double CFL) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int f = tile * 256 + threadIdx.x;
+    double dtf = 1e5;
+    if (f < p.nf) {
+        const int2 oc = p.fcell[f];
+        const double4 g0 = p.fgeo[2 * f], g1 = p.fgeo[2 * f + 1];
+        const double Sfx = g0.x, Sfy = g0.y, magSf = g0.z, deltaR = g0.w;
+        const double fraction = afar_hypot(g1.x, g1.y) / deltaR;
+        const typename VT<R>::v4 wo = p.W[oc.x], wn = p.W[oc.y];
+        const double uxo = wo.y, uyo = wo.z, To = wo.w, uxn = wn.y, uyn = wn.z, Tn = wn.w;
+        dtf = CFL * deltaR * magSf /
+              ((1.0 - fraction) * fabs(uxo * Sfx + uyo * Sfy) + fraction * fabs(uxn * Sfx + uyn * Sfy) +
+               sqrt((R_ / Cv + 1.0) * R_ * ((1.0 - fraction) * To + fraction * Tn)) * magSf);
+    }
+    // wavefront min through cross-lane swizzles, then across the 4 waves
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(dtf, off);
+        dtf = dtf < o ? dtf : o;
+    }
+    __shared__ double wmin[4];
+    __shared__ int last;
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dtf;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < 4; ++w) m = m < wmin[w] ? m : wmin[w];
+        atomicMin(p.dt_accum, (unsigned long long)afar_bits(m));
+        __threadfence();
+        const unsigned t = atomicAdd(p.dt_count, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        const unsigned long long v = atomicExch(p.dt_accum, (unsigned long long)afar_bits(1e5));
+        *p.dt = afar_from_bits(v);
+        atomicExch(p.dt_count, 0u);
+    }
+}
+
+// ---------------------------------------------------------------- K5 reconstruct
+
+template <typename R>
+__device__ __forceinline__ void limiter_process(R& lim, R& phif, R phic, R phiMax, R phiMin, R gx, R gy, R dx, R dy) {
+    // cudaLimiter.h:35-50
+    phif = phic + (gx * dx + gy * dy);
+    const R d2 = phif - phic;
+    const R d1 = (R)(phif < phic) * (phiMin - phic) + (R)(phif >= phic) * (phiMax - phic);
+    R yy = fmin((R)2, ((R)0.5 * d1) / ((d2 + (R)(d2 >= (R)0) * K<R>::VSMALL) - (R)(d2 < (R)0) * K<R>::VSMALL));
+    const int valid = fabs(d2) >= K<R>::SMALL;
+    yy = (R)valid * yy + (R)(1 - valid) * (R)1;
+    lim = fmin(lim, yy);
+}
+
+template <typename R, int TILE>
+__global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
+    using v4 = typename VT<R>::v4;
+    using v2 = typename VT<R>::v2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    v4* sW = reinterpret_cast<v4*>(smem_raw);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= p.ntiles) return;
+    const int tid = threadIdx.x, c = tile * TILE + tid;
+    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
+    if (c < p.nc) sW[tid] = p.W[c];
+    for (int i = tid; i < hn; i += TILE) sW[TILE + i] = p.W[p.halo[hb + i]];
+    __syncthreads();
+    if (c >= p.nc) return;
+    const int S = p.scnt[c], C = p.ccnt[c];
+    const int maxS = p.maxS, maxC = p.maxC, KR = maxC * (maxS + 1);
+    const v4 wc = sW[tid];
+
+    R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
+    for (int i = 0; i < C; ++i) {
+        const v2 M = p.lsq[sell(c, i, maxC)];
+        const v4 wj = sW[p.sten[sell(c, i, maxS)]];
+        R dphi = wj.x - wc.x;
+        gpx += M.x * dphi;
+        gpy += M.y * dphi;
+        dphi = wj.y - wc.y;
+        guxx += M.x * dphi;
+        guyx += M.y * dphi;
+        dphi = wj.z - wc.z;
+        guxy += M.x * dphi;
+        guyy += M.y * dphi;
+        dphi = wj.w - wc.w;
+        gTx += M.x * dphi;
+        gTy += M.y * dphi;
+    }
+    p.gp[c] = v2{gpx, gpy};
+    p.gu[c] = v2{guxx, guyx};
+    p.gv[c] = v2{guxy, guyy};
+    p.gT[c] = v2{gTx, gTy};
+
+    R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z, TMax = wc.w, TMin = wc.w;
+    for (int i = 0; i < C; ++i) {
+        const v4 wj = sW[p.sten[sell(c, i, maxS)]];
+        pMax = fmax(wj.x, pMax);
+        pMin = fmin(wj.x, pMin);
+        uxMax = fmax(wj.y, uxMax);
+        uxMin = fmin(wj.y, uxMin);
+        uyMax = fmax(wj.z, uyMax);
+        uyMin = fmin(wj.z, uyMin);
+        TMax = fmax(wj.w, TMax);
+        TMin = fmin(wj.w, TMin);
+    }
+
+    const int nf = p.nf;
+    if (fabs(TMax - TMin) < K<R>::VSMALL) {
+        p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
+        for (int fi = 0; fi < C; ++fi) {
+            const int sf = p.slot_face[sell(c, fi, maxC)];
+            const int isNei = sf < 0;
+            const int f = (sf < 0 ? -sf : sf) - 1;
+            if (f >= nf) continue;
+            const double2 d = p.slot_d[sell(c, fi, maxC)];
+            const R dx = (R)d.x, dy = (R)d.y;
+            v4 o;
+            o.x = wc.x + gpx * dx + gpy * dy;
+            o.y = wc.y + guxx * dx + guyx * dy;
+            o.z = wc.z + guxy * dx + guyy * dy;
+            o.w = wc.w + gTx * dx + gTy * dy;
+            p.fv[2 * f + isNei] = o;
+        }
+        p.smooth[c] = 0;
+        return;
+    }
+
+    R limP = 2, limUx = 2, limUy = 2, limT = 2;
+    R smoothness = 0;
+    int isNotTrouble = 1;
+    const R rP = fabs(pMax - pMin), rUx = fabs(uxMax - uxMin), rUy = fabs(uyMax - uyMin), rT = fabs(TMax - TMin);
+    for (int fi = 0; fi < C; ++fi) {
+        const int sf = p.slot_face[sell(c, fi, maxC)];
+        const int isNei = sf < 0;
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        if (f >= nf) continue;
+        const double2 d = p.slot_d[sell(c, fi, maxC)];
+        const R dx = (R)d.x, dy = (R)d.y;
+        const double2 e = p.slot_e[sell(c, fi, maxC)];
+        const R deltaR = (R)afar_hypot(e.x, e.y);
+        const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
+        const int nbl = p.sten[sell(c, fi, maxS)];
+
+        const R* a = p.rbf + sell(c, fi * (maxS + 1), KR);  // row stride 64
+        R coef = a[0];
+        R pf = coef * wc.x, uxf = coef * wc.y, uyf = coef * wc.z, Tf = coef * wc.w;
+        for (int ci = 0; ci < S; ++ci) {
+            const v4 wj = sW[p.sten[sell(c, ci, maxS)]];
+            coef = a[(size_t)(ci + 1) * 64];
+            pf += coef * wj.x;
+            uxf += coef * wj.y;
+            uyf += coef * wj.z;
+            Tf += coef * wj.w;
+        }
+        v4 o;
+        R phif;
+        limiter_process(limP, phif, wc.x, pMax, pMin, gpx, gpy, dx, dy);
+        o.x = (R)(rP >= K<R>::SMALL) * pf + (R)(rP < K<R>::SMALL) * phif;
+        isNotTrouble = phif > (R)0;
+        smoothness = fmax(smoothness, fabs(((R)1 - fraction) * wc.x + fraction * sW[nbl].x - phif) / deltaR +
+                                          (R)(pf <= (R)0) * (R)1e20 + (R)(phif <= (R)0) * (R)1e20);
+        limiter_process(limUx, phif, wc.y, uxMax, uxMin, guxx, guyx, dx, dy);
+        o.y = (R)(rUx >= K<R>::SMALL) * uxf + (R)(rUx < K<R>::SMALL) * phif;
+        limiter_process(limUy, phif, wc.z, uyMax, uyMin, guxy, guyy, dx, dy);
+        o.z = (R)(rUy >= K<R>::SMALL) * uyf + (R)(rUy < K<R>::SMALL) * phif;
+        limiter_process(limT, phif, wc.w, TMax, TMin, gTx, gTy, dx, dy);
+        o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
+        isNotTrouble = phif > (R)0;
+        p.fv[2 * f + isNei] = o;
+    }
+    const R nt = (R)isNotTrouble;
+    p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
+    smoothness = smoothness / ((R)0.5 * (pMax + pMin) * nt + K<R>::VSMALL);
+    p.smooth[c] = smoothness > (R)1;
+}
+
+// ---------------------------------------------------------------- K6 BVD indicator
+
+template <typename R> __device__ __forceinline__ R to_bj(R y) { return fmin((R)2 * y, (R)1); }
+template <typename R> __device__ __forceinline__ R to_venkat(R y) {
+    return fmin((y * y + y + K<R>::VSMALL) / (y * y + (R)0.5 + (R)0.5 * y + K<R>::VSMALL), (R)1);
+}
+template <typename R> __device__ __forceinline__ R transform_limiter(R lim, int isShock) {
+    return (R)isShock * to_venkat(lim) + (R)(1 - isShock) * to_bj(lim);
+}
+
+template <typename R, int TILE>
+__global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
+    using v4 = typename VT<R>::v4;
+    using v2 = typename VT<R>::v2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    int8_t* sS = reinterpret_cast<int8_t*>(smem_raw);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= p.ntiles) return;
+    const int tid = threadIdx.x, c = tile * TILE + tid;
+    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
+    if (c < p.nc) sS[tid] = p.smooth[c];
+    for (int i = tid; i < hn; i += TILE) sS[TILE + i] = p.smooth[p.halo[hb + i]];
+    __syncthreads();
+    if (c >= p.nc) return;
+    const int S = p.scnt[c], C = p.ccnt[c];
+    const int maxS = p.maxS, maxC = p.maxC, nf = p.nf;
+    int8_t shockType = sS[tid];
+    int8_t shockTypeExt = sS[tid];
+    for (int ci = 0; ci < C; ++ci) {
+        shockType += sS[p.sten[sell(c, ci, maxS)]];
+        shockTypeExt = shockType;
+    }
+    for (int ci = C; ci < S; ++ci) shockTypeExt += sS[p.sten[sell(c, ci, maxS)]];
+
+    const v4 limc = p.lim[c];
+    const v2 gT = p.gT[c];
+    const R Tc = p.W[c].w;
+    int bvd = 0, bfaces = 0;
+    for (int fi = 0; fi < C; ++fi) {
+        const int sf = p.slot_face[sell(c, fi, maxC)];
+        const int isNei = sf < 0;
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        if (f >= nf) {
+            bfaces += 1;
+            if (bfaces >= 2) {  // corner cells use the second-order reconstruction
+                bvd = 1;
+                break;
+            }
+            continue;
+        }
+        const double2 d = p.slot_d[sell(c, fi, maxC)];
+        const R dx = (R)d.x, dy = (R)d.y;
+        const R TfOwn = p.fv[2 * f].w, TfNei = p.fv[2 * f + 1].w;
+        const R Tf_cur = (R)isNei * TfNei + (R)(1 - isNei) * TfOwn;
+        const R Tf_nei = (R)(1 - isNei) * TfNei + (R)isNei * TfOwn;
+        const R lim = to_bj(limc.w);
+        const R Tf_cur_TVD = Tc + lim * (gT.x * dx + gT.y * dy);
+        const int typeval = fabs(Tf_cur_TVD - Tf_nei) < K<R>::BVD_FACTOR * fabs(Tf_cur - Tf_nei);
+        const int isNotFlat = fabs(Tf_cur - Tf_nei) > K<R>::VSMALL;
+        const int isLimited = (lim < K<R>::VALID_LIM_FACTOR) + (shockTypeExt > 0);
+        bvd = max(bvd, isNotFlat * isLimited * typeval);
+    }
+    const int finalType = (shockType > 0) * (shockTypeExt > 1) * (-2) + (bvd > 0);
+    p.ftype[c] = (int8_t)finalType;
+    const int isShock = finalType < 0;
+    p.lim[c] = v4{transform_limiter(limc.x, isShock), transform_limiter(limc.y, isShock),
+                  transform_limiter(limc.z, isShock), transform_limiter(limc.w, isShock)};
+}
+
+// ---------------------------------------------------------------- K7 flux
+
+template <typename R>
+__device__ __forceinline__ R minimize_delta(R L, R Rv, R LT, R RT) {
+    // cudaRiemannSolver.h:7-14
+    R d = fabs(Rv - L);
+    d = fmin(d, fabs(Rv - LT));
+    d = fmin(d, fabs(RT - L));
+    d = fmin(d, fabs(RT - LT));
+    return (R)(Rv >= L) * d - (R)(Rv < L) * d;
+}
+
+template <typename R>
+__device__ __forceinline__ void kfvs_flux(R& rhoFlux, R& rhoUxFlux, R& rhoUyFlux, R& rhoEFlux, R rhoL, R rhoR,
+                                          R pL, R pR, R uxL, R uxR, R uyL, R uyR, R TL, R TR, R Rg, R Cv, R magSf,
+                                          R Sfx, R Sfy) {
+    // cudaRiemannSolver.h:16-68
+    const R nfx = Sfx / magSf, nfy = Sfy / magSf;
+    const R uxLl = uxL * nfx + uyL * nfy, uyLl = uyL * nfx - uxL * nfy;
+    const R uxRl = uxR * nfx + uyR * nfy, uyRl = uyR * nfx - uxR * nfy;
+    const R lamL = (R)0.5 * rhoL / pL, lamR = (R)0.5 * rhoR / pR;
+    const R PI = (R)3.141593;
+    const R Mu0p = (R)0.5 * kerfc(-sqrt(lamL) * uxLl);
+    const R Mu1p = uxLl * Mu0p + (R)0.5 * kexp(-lamL * uxLl * uxLl) / (sqrt(PI * lamL));
+    const R Mu2p = uxLl * Mu1p + (R)0.5 * Mu0p / lamL;
+    const R Mu3p = uxLl * Mu2p + Mu1p / lamL;
+    const R Mu0n = (R)0.5 * kerfc(sqrt(lamR) * uxRl);
+    const R Mu1n = uxRl * Mu0n - (R)0.5 * kexp(-lamR * uxRl * uxRl) / (sqrt(PI * lamR));
+    const R Mu2n = uxRl * Mu1n + (R)0.5 * Mu0n / lamR;
+    const R Mu3n = uxRl * Mu2n + Mu1n / lamR;
+    const R Mv1L = uyLl, Mv1R = uyRl;
+    const R Kd = (R)2 * Cv / Rg - (R)2;
+    const R trho = rhoL * Mu1p + rhoR * Mu1n;
+    const R trhoUx = rhoL * Mu2p + rhoR * Mu2n;
+    const R trhoUy = rhoL * Mu1p * Mv1L + rhoR * Mu1n * Mv1R;
+    const R trhoE = (R)0.5 * rhoL * (Mu3p + Mu1p * (uyLl * Mv1L + (R)0.5 / lamL) + Mu1p * Kd * (R)0.5 / lamL) +
+                    (R)0.5 * rhoR * (Mu3n + Mu1n * (uyRl * Mv1R + (R)0.5 / lamR) + Mu1n * Kd * (R)0.5 / lamR);
+    rhoUxFlux += trhoUx * Sfx - trhoUy * Sfy - (R)0.5 * (rhoFlux * uxL + pL * Sfx) - (R)0.5 * (rhoFlux * uxR + pR * Sfx);
+    rhoUyFlux += trhoUx * Sfy + trhoUy * Sfx - (R)0.5 * (rhoFlux * uyL + pL * Sfy) - (R)0.5 * (rhoFlux * uyR + pR * Sfy);
+    rhoEFlux += trhoE * magSf - (R)0.5 * rhoFlux * (Cv * TL + (R)0.5 * (uxL * uxL + uyL * uyL) + pL / rhoL) -
+                (R)0.5 * rhoFlux * (Cv * TR + (R)0.5 * (uxR * uxR + uyR * uyR) + pR / rhoR);
+    rhoFlux += trho * magSf - (R)0.5 * rhoL * (uxL * Sfx + uyL * Sfy) - (R)0.5 * rhoR * (uxR * Sfx + uyR * Sfy);
+}
+
+template <typename R>
+__device__ __forceinline__ void roe_flux(R& rhoFlux, R& rhoUxFlux, R& rhoUyFlux, R& rhoEFlux, R rhoL, R rhoR, R pL,
+                                         R pR, R uxL, R uxR, R uyL, R uyR, R TL, R TR, R dP, R dRho, R gamma, R Cv,
+                                         R magSf, R Sfx, R Sfy) {
+    // cudaRiemannSolver.h:71-136
+    const R nfx = Sfx / magSf, nfy = Sfy / magSf;
+    const R VL = uxL * nfx + uyL * nfy, VR = uxR * nfx + uyR * nfy;
+    const R hL = Cv * TL + (R)0.5 * (uxL * uxL + uyL * uyL) + pL / rhoL;
+    const R hR = Cv * TR + (R)0.5 * (uxR * uxR + uyR * uyR) + pR / rhoR;
+    const R rhoT = sqrt(fmax(rhoL * rhoR, (R)1e-20));
+    const R wL = sqrt(fmax(rhoL, (R)1e-20)) / (sqrt(fmax(rhoL, (R)1e-20)) + sqrt(fmax(rhoR, (R)1e-20)));
+    const R wR = (R)1 - wL;
+    const R uxT = uxL * wL + uxR * wR, uyT = uyL * wL + uyR * wR, hT = hL * wL + hR * wR;
+    const R q2 = uxT * uxT + uyT * uyT;
+    const R cT = sqrt(fmax((gamma - (R)1) * (hT - (R)0.5 * q2), (R)1e-20));
+    const R VT_ = uxT * nfx + uyT * nfy;
+    const R dUx = uxR - uxL, dUy = uyR - uyL;
+    const R dV = dUx * nfx + dUy * nfy;
+    const R r1 = (dP - rhoT * cT * dV) / ((R)2 * cT * cT);
+    const R r2 = dRho - dP / (cT * cT);
+    const R r3 = (dP + rhoT * cT * dV) / ((R)2 * cT * cT);
+    const R cL = sqrt(fmax((gamma - (R)1) * (hL - (R)0.5 * (uxL * uxL + uyL * uyL)), (R)1e-20));
+    const R cR = sqrt(fmax((gamma - (R)1) * (hR - (R)0.5 * (uxR * uxR + uyR * uyR)), (R)1e-20));
+    R eps = (R)2 * fmax((R)0, (VR - cR) - (VL - cL));
+    R l1 = fabs(VT_ - cT);
+    l1 = (R)(l1 >= eps) * l1 + (R)(l1 < eps) * ((l1 * l1 + eps * eps) / ((R)2 * eps + (R)1e-30));
+    eps = (R)2 * fmax((R)0, VR - VL);
+    R l2 = fabs(VT_);
+    l2 = (R)(l2 >= eps) * l2 + (R)(l2 < eps) * ((l2 * l2 + eps * eps) / ((R)2 * eps + (R)1e-30));
+    eps = (R)2 * fmax((R)0, (VR + cR) - (VL + cL));
+    R l3 = fabs(VT_ + cT);
+    l3 = (R)(l3 >= eps) * l3 + (R)(l3 < eps) * ((l3 * l3 + eps * eps) / ((R)2 * eps + (R)1e-30));
+    rhoFlux -= (R)0.5 * (l1 * r1 + l2 * r2 + l3 * r3) * magSf;
+    rhoUxFlux -= (R)0.5 * (l1 * r1 * (uxT - cT * nfx) + l2 * (r2 * uxT + rhoT * (dUx - dV * nfx)) + l3 * r3 * (uxT + cT * nfx)) * magSf;
+    rhoUyFlux -= (R)0.5 * (l1 * r1 * (uyT - cT * nfy) + l2 * (r2 * uyT + rhoT * (dUy - dV * nfy)) + l3 * r3 * (uyT + cT * nfy)) * magSf;
+    rhoEFlux -= (R)0.5 * (l1 * r1 * (hT - cT * VT_) + l2 * (r2 * (R)0.5 * q2 + rhoT * (uxT * dUx + uyT * dUy - VT_ * dV)) +
+                          l3 * r3 * (hT + cT * VT_)) * magSf;
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double Cv) {
+    using v4 = typename VT<R>::v4;
+    using v2 = typename VT<R>::v2;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int i = tile * 256 + threadIdx.x;
+    const int nf = p.nf;
+    if (i < nf) {
+        const int f = i;
+        const int2 oc = p.fcell[f];
+        const int own = oc.x, nei = oc.y;
+        const double4 g0 = p.fgeo[2 * f], g1 = p.fgeo[2 * f + 1];
+        const R dxL = (R)g1.x, dyL = (R)g1.y, dxR = (R)g1.z, dyR = (R)g1.w;
+        const int8_t to = p.ftype[own], tn = p.ftype[nei];
+        const int isTVD_L = to != 0, isTVD_R = tn != 0;
+        const int isShock = to < 0 || tn < 0;
+        const R magSf = (R)g0.z, Sfx = (R)g0.x, Sfy = (R)g0.y;
+        const R Rg = (R)Rgas;
+        const v4 wL = p.W[own], wR = p.W[nei];
+        const v4 lL = p.lim[own], lR = p.lim[nei];
+        const v2 gpL = p.gp[own], guL = p.gu[own], gvL = p.gv[own], gTL = p.gT[own];
+        const v2 gpR = p.gp[nei], guR = p.gu[nei], gvR = p.gv[nei], gTR = p.gT[nei];
+        const v4 fL = p.fv[2 * f], fR = p.fv[2 * f + 1];
+
+        const R pLT = wL.x + lL.x * (gpL.x * dxL + gpL.y * dyL);
+        const R uxLT = wL.y + lL.y * (guL.x * dxL + guL.y * dyL);
+        const R uyLT = wL.z + lL.z * (gvL.x * dxL + gvL.y * dyL);
+        const R TLT = wL.w + lL.w * (gTL.x * dxL + gTL.y * dyL);
+        const R rhoLT = pLT / (Rg * TLT);
+        const R pRT = wR.x + lR.x * (gpR.x * dxR + gpR.y * dyR);
+        const R uxRT = wR.y + lR.y * (guR.x * dxR + guR.y * dyR);
+        const R uyRT = wR.z + lR.z * (gvR.x * dxR + gvR.y * dyR);
+        const R TRT = wR.w + lR.w * (gTR.x * dxR + gTR.y * dyR);
+        const R rhoRT = pRT / (Rg * TRT);
+
+        const R pL = (R)(1 - isTVD_L) * fL.x + (R)isTVD_L * pLT;
+        const R uxL = (R)(1 - isTVD_L) * fL.y + (R)isTVD_L * uxLT;
+        const R uyL = (R)(1 - isTVD_L) * fL.z + (R)isTVD_L * uyLT;
+        const R TL = (R)(1 - isTVD_L) * fL.w + (R)isTVD_L * TLT;
+        const R rhoL = pL / (Rg * TL);
+        const R pR = (R)(1 - isTVD_R) * fR.x + (R)isTVD_R * pRT;
+        const R uxR = (R)(1 - isTVD_R) * fR.y + (R)isTVD_R * uxRT;
+        const R uyR = (R)(1 - isTVD_R) * fR.z + (R)isTVD_R * uyRT;
+        const R TR = (R)(1 - isTVD_R) * fR.w + (R)isTVD_R * TRT;
+        const R rhoR = pR / (Rg * TR);
+
+        const R dP = minimize_delta(pL, pR, pLT, pRT);
+        const R dRho = minimize_delta(rhoL, rhoR, rhoLT, rhoRT);
+
+        const R CvR = (R)Cv;
+        R rhoFlux = (R)0.5 * rhoL * (uxL * Sfx + uyL * Sfy) + (R)0.5 * rhoR * (uxR * Sfx + uyR * Sfy);
+        R rhoUxFlux = (R)0.5 * (rhoFlux * uxL + pL * Sfx) + (R)0.5 * (rhoFlux * uxR + pR * Sfx);
+        R rhoUyFlux = (R)0.5 * (rhoFlux * uyL + pL * Sfy) + (R)0.5 * (rhoFlux * uyR + pR * Sfy);
+        R rhoEFlux = (R)0.5 * rhoFlux * (CvR * TL + (R)0.5 * (uxL * uxL + uyL * uyL) + pL / rhoL) +
+                     (R)0.5 * rhoFlux * (CvR * TR + (R)0.5 * (uxR * uxR + uyR * uyR) + pR / rhoR);
+        if (isShock)
+            kfvs_flux<R>(rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Rg,
+                         CvR, magSf, Sfx, Sfy);
+        else
+            roe_flux<R>(rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, dP,
+                        dRho, (R)((Cv + Rgas) / Cv), CvR, magSf, Sfx, Sfy);
+        p.flux[f] = double4{(double)rhoFlux, (double)rhoUxFlux, (double)rhoUyFlux, (double)rhoEFlux};
+    } else if (i < nf + p.nb) {
+        // boundary faces: physical flux of the ghost state (cudaRiemannSolver.h:254-270)
+        const int b = i - nf;
+        const double4 g = p.bgeo[b];
+        const typename VT<R>::v4 w = p.W[p.nc + b];
+        const double bp = w.x, bUx = w.y, bUy = w.z, bT = w.w;
+        const double rho = bp / (Rgas * bT);
+        const double rhoFlux = rho * (bUx * g.x + bUy * g.y);
+        p.flux[i] = double4{rhoFlux, rhoFlux * bUx + bp * g.x, rhoFlux * bUy + bp * g.y,
+                            rhoFlux * (Cv * bT + 0.5 * (bUx * bUx + bUy * bUy) + bp / rho)};
+    }
+}
+
+// ---------------------------------------------------------------- K8-K11 update
+
+__device__ __forceinline__ double4 boundary_value(int label, double4 w, double4 g, double4 val) {
+    // src/CUDA/BCs/2D/*.h, dispatch table cudaEulerBCs.h:34-39; label 6 = per-patch fixed value
+    switch (label) {
+        case 0: return double4{w.x, 0.0, 0.0, 1.0};
+        case 1: return double4{w.x, 0.0, 0.0, w.w};
+        case 2: {
+            const double nfx = g.x / g.z, nfy = g.y / g.z;
+            return double4{w.x, w.y - (w.y * nfx + w.z * nfy) * nfx, w.z - (w.y * nfx + w.z * nfy) * nfy, w.w};
+        }
+        case 3: return double4{1.0, 3.0, 0.0, 1.0};
+        case 4: return w;
+        case 5: return double4{30.05945, 4.07795, 0.0, 5.97677};
+        default: return val;
+    }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_update(DevPtrs<R> p, double Rgas, double gamma, double dt_val,
+                                                 const double* dt_ptr, double b1, double b2, double b3, int last) {
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int c = tile * 256 + threadIdx.x;
+    if (c >= p.nc) return;
+    const int C = p.ccnt[c], maxC = p.maxC, nf = p.nf;
+    const double dt = dt_ptr ? *dt_ptr : dt_val;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    for (int fi = 0; fi < C; ++fi) {
+        const int sf = p.slot_face[sell(c, fi, maxC)];
+        const int dir = (sf > 0) - (sf < 0);
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        const double4 F = p.flux[f];
+        r0 += dir * F.x;
+        r1 += dir * F.y;
+        r2 += dir * F.z;
+        r3 += dir * F.w;
+    }
+    const double V = p.vol[c];
+    const double4 uo = p.Uold[c], u = p.U[c];
+    const double rho = b1 * uo.x + b2 * u.x + b3 * dt * (-r0 / V);
+    const double rhoUx = b1 * uo.y + b2 * u.y + b3 * dt * (-r1 / V);
+    const double rhoUy = b1 * uo.z + b2 * u.z + b3 * dt * (-r2 / V);
+    const double rhoE = b1 * uo.w + b2 * u.w + b3 * dt * (-r3 / V);
+    const double4 un = double4{rho, rhoUx, rhoUy, rhoE};
+    p.U[c] = un;
+    if (last) p.Uold[c] = un;
+    const double pr = (gamma - 1.0) * (rhoE - 0.5 * (rhoUx * rhoUx + rhoUy * rhoUy) / rho);
+    const double4 w = double4{pr, rhoUx / rho, rhoUy / rho, pr / (Rgas * rho)};
+    typename VT<R>::v4 wr;
+    wr.x = (R)w.x;
+    wr.y = (R)w.y;
+    wr.z = (R)w.z;
+    wr.w = (R)w.w;
+    p.W[c] = wr;
+    // ghost states of this cell's boundary faces (K10), from the updated state
+    for (int fi = 0; fi < C; ++fi) {
+        const int sf = p.slot_face[sell(c, fi, maxC)];
+        const int f = sf - 1;
+        if (f < nf) continue;
+        const int b = f - nf;
+        const double4 g = boundary_value(p.blabel[b], w, p.bgeo[b], p.bval[b]);
+        typename VT<R>::v4 gr;
+        gr.x = (R)g.x;
+        gr.y = (R)g.y;
+        gr.z = (R)g.z;
+        gr.w = (R)g.w;
+        p.W[p.nc + b] = gr;
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+
+template <typename R>
+void launch_lsq(const DevPtrs<R>& p, hipStream_t s) {
+    if (p.nc == 0) return;
+    hipLaunchKernelGGL(k_lsq<R>, dim3((p.nc + 255) / 256), dim3(256), 0, s, p);
+}
+
+static inline int pad8(int n) { return ((n + 7) / 8) * 8; }
+
+template <typename R>
+void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s) {
+    const int nt = pad8(std::max(1, (p.nf + 255) / 256));
+    hipLaunchKernelGGL(k_cfl<R>, dim3(nt), dim3(256), 0, s, p, Rg, Cv, CFL);
+}
+
+template <typename R>
+void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s) {
+    const int nb = pad8(p.ntiles);
+    switch (tile) {
+        case 64: hipLaunchKernelGGL((k_reconstruct<R, 64>), dim3(nb), dim3(64), lds, s, p); break;
+        case 128: hipLaunchKernelGGL((k_reconstruct<R, 128>), dim3(nb), dim3(128), lds, s, p); break;
+        default: hipLaunchKernelGGL((k_reconstruct<R, 256>), dim3(nb), dim3(256), lds, s, p); break;
+    }
+}
+
+template <typename R>
+void launch_bvd(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s) {
+    const int nb = pad8(p.ntiles);
+    switch (tile) {
+        case 64: hipLaunchKernelGGL((k_bvd<R, 64>), dim3(nb), dim3(64), lds, s, p); break;
+        case 128: hipLaunchKernelGGL((k_bvd<R, 128>), dim3(nb), dim3(128), lds, s, p); break;
+        default: hipLaunchKernelGGL((k_bvd<R, 256>), dim3(nb), dim3(256), lds, s, p); break;
+    }
+}
+
+template <typename R>
+void launch_flux(const DevPtrs<R>& p, double Rg, double Cv, hipStream_t s) {
+    const int nt = pad8(std::max(1, (p.nf + p.nb + 255) / 256));
+    hipLaunchKernelGGL(k_flux<R>, dim3(nt), dim3(256), 0, s, p, Rg, Cv);
+}
+
+template <typename R>
+void launch_update(const DevPtrs<R>& p, double Rg, double gamma, double dt, const double* dt_ptr, double b1,
+                   double b2, double b3, int last, hipStream_t s) {
+    const int nt = pad8(std::max(1, (p.nc + 255) / 256));
+    hipLaunchKernelGGL(k_update<R>, dim3(nt), dim3(256), 0, s, p, Rg, gamma, dt, dt_ptr, b1, b2, b3, last);
+}
+
+template <typename R>
+hipError_t set_lds_limits(size_t lds) {
+    hipError_t e = hipSuccess;
+    const void* fns[] = {(const void*)k_reconstruct<R, 64>, (const void*)k_reconstruct<R, 128>,
+                         (const void*)k_reconstruct<R, 256>};
+    for (const void* fn : fns) {
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    return e;
+}
+
+#define AFAR_INSTANTIATE(R)                                                                                        \
+    template void launch_lsq<R>(const DevPtrs<R>&, hipStream_t);                                                  \
+    template void launch_cfl<R>(const DevPtrs<R>&, double, double, double, hipStream_t);                          \
+    template void launch_reconstruct<R>(const DevPtrs<R>&, int, size_t, hipStream_t);                             \
+    template void launch_bvd<R>(const DevPtrs<R>&, int, size_t, hipStream_t);                                     \
+    template void launch_flux<R>(const DevPtrs<R>&, double, double, hipStream_t);                                 \
+    template void launch_update<R>(const DevPtrs<R>&, double, double, double, const double*, double, double,      \
+                                   double, int, hipStream_t);                                                     \
+    template hipError_t set_lds_limits<R>(size_t);
+
+AFAR_INSTANTIATE(double)
+AFAR_INSTANTIATE(float)
+
+// Self-test entry: evaluates the shared elementary functions on the device.
+__global__ void k_math_selftest(const double* x, double* out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = x[i];
+    out[4 * i] = afar_exp(v);
+    out[4 * i + 1] = afar_erfc(v);
+    out[4 * i + 2] = sqrt(v < 0 ? -v : v);
+    out[4 * i + 3] = 1.0 / v;
+}
+
+void launch_math_selftest(const double* x, double* out, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_math_selftest, dim3((n + 255) / 256), dim3(256), 0, s, x, out, n);
+}
+
+}  // namespace afar

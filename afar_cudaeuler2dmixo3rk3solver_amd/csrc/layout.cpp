@@ -1,0 +1,320 @@
+// Reference-layout host arrays and the gfx950 device layout built from them.
+#include "layout.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <unordered_map>
+
+namespace afar {
+
+void conserved_from_primitive(RefArrays& ra, double R, double Cv) {
+    // basicCUDAsolver.cpp:220-223
+    ra.cFD.assign(4 * (size_t)ra.nc, 0.0);
+    for (int i = 0; i < ra.nc; ++i) {
+        const double p = ra.FD[4 * i], ux = ra.FD[4 * i + 1], uy = ra.FD[4 * i + 2], T = ra.FD[4 * i + 3];
+        const double rho = p / (R * T);
+        ra.cFD[4 * i] = rho;
+        ra.cFD[4 * i + 1] = rho * ux;
+        ra.cFD[4 * i + 2] = rho * uy;
+        ra.cFD[4 * i + 3] = rho * (Cv * T + 0.5 * (ux * ux + uy * uy));
+    }
+}
+
+void apply_bc_host(RefArrays& ra) {
+    // Same formulas as the device boundary kernel (src/CUDA/BCs/2D/*.h).
+    for (int b = 0; b < ra.nb; ++b) {
+        const int c = ra.boundaryFaceNeiLabel[b];
+        const double* w = &ra.FD[4 * (size_t)c];
+        double* g = &ra.FD[4 * ((size_t)ra.nc + b)];
+        const double* F = &ra.FACE[5 * ((size_t)ra.nf + b)];
+        const double magSf = F[2], Sfx = F[3], Sfy = F[4];
+        switch (ra.boundaryFacesType[b]) {
+            case 0: g[0] = w[0]; g[1] = 0.0; g[2] = 0.0; g[3] = 1.0; break;
+            case 1: g[0] = w[0]; g[1] = 0.0; g[2] = 0.0; g[3] = w[3]; break;
+            case 2: {
+                const double nfx = Sfx / magSf, nfy = Sfy / magSf;
+                g[0] = w[0];
+                g[1] = w[1] - (w[1] * nfx + w[2] * nfy) * nfx;
+                g[2] = w[2] - (w[1] * nfx + w[2] * nfy) * nfy;
+                g[3] = w[3];
+                break;
+            }
+            case 3: g[0] = 1.0; g[1] = 3.0; g[2] = 0.0; g[3] = 1.0; break;
+            case 4: g[0] = w[0]; g[1] = w[1]; g[2] = w[2]; g[3] = w[3]; break;
+            case 5: g[0] = 30.05945; g[1] = 4.07795; g[2] = 0.0; g[3] = 5.97677; break;
+            case 6: for (int k = 0; k < 4; ++k) g[k] = ra.BVALUE[4 * (size_t)b + k]; break;
+            default: throw Error(1, "boundary label out of range");
+        }
+    }
+}
+
+RefArrays make_ref_arrays(const Mesh& m, const Precomputed& pc, const double* W0, double R, double Cv) {
+    RefArrays ra;
+    const int nc = m.nc, nf = m.nf, nb = m.nb;
+    const Stencils& st = pc.st;
+    ra.nc = nc;
+    ra.nf = nf;
+    ra.nb = nb;
+    ra.maxS = st.maxS;
+    ra.maxC = st.maxC;
+    ra.CELL.resize(2 * (size_t)(nc + nb));
+    for (int i = 0; i < nc + nb; ++i) {
+        ra.CELL[2 * i] = m.C[i].x;
+        ra.CELL[2 * i + 1] = m.C[i].y;
+    }
+    ra.FACE.resize(5 * (size_t)(nf + nb));
+    for (int f = 0; f < nf + nb; ++f) {
+        double* F = &ra.FACE[5 * (size_t)f];
+        F[0] = m.Cf[f].x;
+        F[1] = m.Cf[f].y;
+        F[2] = m.magSf[f];
+        F[3] = m.Sf[f].x;
+        F[4] = m.Sf[f].y;
+    }
+    ra.cellVolume = m.V;
+    ra.FD.assign(4 * (size_t)(nc + nb), 0.0);
+    for (size_t i = 0; i < 4 * (size_t)nc; ++i) ra.FD[i] = W0[i];
+    ra.neighbour.resize(2 * (size_t)nf);
+    for (int f = 0; f < nf; ++f) {
+        ra.neighbour[2 * f] = m.own[f];
+        ra.neighbour[2 * f + 1] = m.nei[f];
+    }
+    for (size_t p = 0; p < m.patches.size(); ++p)
+        if (m.patch_size[p] > 0) ra.boundaryFacesNum.push_back(m.patch_size[p]);
+    ra.npatch = (int)ra.boundaryFacesNum.size();
+    ra.boundaryFaceNeiLabel.resize(nb);
+    ra.boundaryFacesType = m.bf_label;
+    ra.BVALUE.assign(4 * (size_t)nb, 0.0);
+    for (int b = 0; b < nb; ++b) {
+        ra.boundaryFaceNeiLabel[b] = m.own[nf + b];
+        for (int k = 0; k < 4; ++k) ra.BVALUE[4 * (size_t)b + k] = m.patches[m.bf_patch[b]].value[k];
+    }
+    ra.stencilSize = st.S;
+    ra.compactStencilSize = st.Cc;
+    ra.stencil.assign((size_t)st.maxS * nc, 0);
+    ra.cellFaces.assign((size_t)st.maxC * nc, 0);
+    for (int c = 0; c < nc; ++c) {
+        for (int i = 0; i < st.S[c]; ++i) ra.stencil[(size_t)st.maxS * c + i] = st.sten[(size_t)c * kMaxValidStencil + i];
+        for (int i = 0; i < st.Cc[c]; ++i)
+            ra.cellFaces[(size_t)st.maxC * c + i] = st.cellFaces[(size_t)c * kMaxValidStencil + i];
+    }
+    ra.RBFbasis = pc.rbf;
+    ra.FSHIFT.resize(2 * (size_t)nf);
+    for (int f = 0; f < nf; ++f) {
+        ra.FSHIFT[2 * f] = m.fshift[f].x;
+        ra.FSHIFT[2 * f + 1] = m.fshift[f].y;
+    }
+    apply_bc_host(ra);
+    conserved_from_primitive(ra, R, Cv);
+    return ra;
+}
+
+BlockTables make_block_tables(const RefArrays& ra) {
+    const int B = 128, nc = ra.nc, maxS = ra.maxS;
+    const int nblk = (nc + B - 1) / B;
+    BlockTables bt;
+    bt.extendStencilSize.assign(nblk, 0);
+    bt.compactExtendStencilSize.assign(nblk, 0);
+    bt.localStencil.assign((size_t)maxS * nc, 0);
+    std::vector<std::vector<int32_t>> ext(nblk);
+    auto outside = [&](int id, int blk) { return id < blk * B || id > B - 1 + blk * B || id >= nc; };
+    for (int blk = 0; blk < nblk; ++blk) {
+        std::unordered_map<int32_t, int32_t> pos;
+        auto& e = ext[blk];
+        for (int pass = 0; pass < 2; ++pass) {
+            for (int t = 0; t < B; ++t) {
+                const int c = t + blk * B;
+                if (c >= nc) break;
+                const int i0 = pass ? ra.compactStencilSize[c] : 0;
+                const int i1 = pass ? ra.stencilSize[c] : ra.compactStencilSize[c];
+                for (int i = i0; i < i1; ++i) {
+                    const int id = ra.stencil[(size_t)maxS * c + i];
+                    if (outside(id, blk) && pos.find(id) == pos.end()) {
+                        pos.emplace(id, (int32_t)e.size());
+                        e.push_back(id);
+                    }
+                }
+            }
+            if (pass == 0) {
+                bt.compactExtendStencilSize[blk] = (uint16_t)e.size();
+                bt.maxCLB = std::max(bt.maxCLB, (int)e.size() + B);
+            }
+        }
+        bt.extendStencilSize[blk] = (uint16_t)e.size();
+        bt.maxLB = std::max(bt.maxLB, (int)e.size() + B);
+        for (int t = 0; t < B; ++t) {
+            const int c = t + blk * B;
+            if (c >= nc) break;
+            for (int i = 0; i < ra.stencilSize[c]; ++i) {
+                const int id = ra.stencil[(size_t)maxS * c + i];
+                bt.localStencil[(size_t)maxS * c + i] =
+                    (uint16_t)(outside(id, blk) ? B + pos[id] : id - blk * B);
+            }
+        }
+    }
+    bt.maxLB = std::max(bt.maxLB, B);
+    bt.maxCLB = std::max(bt.maxCLB, B);
+    const int stride = bt.maxLB - B;
+    bt.extendStencil.assign((size_t)std::max(stride, 1) * nblk, 0);
+    for (int blk = 0; blk < nblk; ++blk)
+        for (size_t i = 0; i < ext[blk].size(); ++i) bt.extendStencil[(size_t)stride * blk + i] = ext[blk][i];
+    return bt;
+}
+
+void global_stencil_from_blocks(int nc, int maxS, int maxLB, const uint8_t* S, const uint16_t* localStencil,
+                                const int32_t* extendStencil, int32_t* stencil) {
+    const int B = 128, stride = maxLB - B;
+    for (int c = 0; c < nc; ++c) {
+        const int blk = c / B;
+        for (int i = 0; i < S[c]; ++i) {
+            const int l = localStencil[(size_t)maxS * c + i];
+            stencil[(size_t)maxS * c + i] = l < B ? blk * B + l : extendStencil[(size_t)stride * blk + (l - B)];
+        }
+    }
+}
+
+DevLayout make_dev_layout(const RefArrays& ra, int tile) {
+    if (tile != 64 && tile != 128 && tile != 256) throw Error(1, "tile must be 64, 128 or 256");
+    DevLayout L;
+    const int nc = ra.nc, nf = ra.nf, nb = ra.nb, maxS = ra.maxS, maxC = ra.maxC;
+    L.nc = nc;
+    L.nf = nf;
+    L.nb = nb;
+    L.maxS = maxS;
+    L.maxC = maxC;
+    L.tile = tile;
+    L.nwaves = (nc + 63) / 64;
+    L.ntiles = (nc + tile - 1) / tile;
+    const size_t nslot = (size_t)L.nwaves * maxC * 64;
+    const int KR = maxC * (maxS + 1);
+    L.scnt = ra.stencilSize;
+    L.ccnt = ra.compactStencilSize;
+    L.sten.assign((size_t)L.nwaves * std::max(maxS, 1) * 64, 0);
+    L.slot_face.assign(nslot, 0);
+    L.slot_d.assign(2 * nslot, 0.0);
+    L.slot_e.assign(2 * nslot, 0.0);
+    L.rbf.assign((size_t)L.nwaves * KR * 64, 0.0);
+    L.vol = ra.cellVolume;
+    const double* CELL = ra.CELL.data();
+    const double* FACE = ra.FACE.data();
+    const double* FS = ra.FSHIFT.data();
+
+    // tile halos and LDS-local stencil ids
+    std::vector<std::vector<int32_t>> halos(L.ntiles);
+    std::vector<int> overflow(L.ntiles, 0);
+    parallel_for(L.ntiles, [&](int64_t t0, int64_t t1) {
+        std::unordered_map<int32_t, int32_t> loc;
+        for (int64_t t = t0; t < t1; ++t) {
+            loc.clear();
+            auto& h = halos[t];
+            const int c0 = (int)t * tile, c1 = std::min(nc, c0 + tile);
+            for (int c = c0; c < c1; ++c)
+                for (int i = 0; i < ra.stencilSize[c]; ++i) {
+                    const int id = ra.stencil[(size_t)maxS * c + i];
+                    int local;
+                    if (id >= c0 && id < c1) {
+                        local = id - c0;
+                    } else {
+                        auto it = loc.find(id);
+                        if (it == loc.end()) {
+                            local = tile + (int)h.size();
+                            loc.emplace(id, local);
+                            h.push_back(id);
+                        } else {
+                            local = it->second;
+                        }
+                    }
+                    if (local > 65535) overflow[t] = 1;
+                    L.sten[L.sell(c, i, maxS)] = (uint16_t)local;
+                }
+        }
+    });
+    for (int t = 0; t < L.ntiles; ++t)
+        if (overflow[t]) throw Error(3, "Too large local block stencil size (LDS ids exceed 16 bits)");
+    L.halo_ptr.assign(L.ntiles + 1, 0);
+    for (int t = 0; t < L.ntiles; ++t) {
+        L.halo_ptr[t + 1] = L.halo_ptr[t] + (int32_t)halos[t].size();
+        L.maxHalo = std::max(L.maxHalo, (int)halos[t].size());
+    }
+    L.halo.resize(L.halo_ptr[L.ntiles]);
+    for (int t = 0; t < L.ntiles; ++t) std::copy(halos[t].begin(), halos[t].end(), L.halo.begin() + L.halo_ptr[t]);
+
+    // slots: face ids, geometry, RBF coefficients
+    parallel_for(nc, [&](int64_t c0, int64_t c1) {
+        for (int64_t cc = c0; cc < c1; ++cc) {
+            const int c = (int)cc;
+            for (int fi = 0; fi < ra.compactStencilSize[c]; ++fi) {
+                const int sf = ra.cellFaces[(size_t)maxC * c + fi];
+                const int f = (sf > 0 ? sf : -sf) - 1;
+                const size_t s = L.sell(c, fi, maxC);
+                L.slot_face[s] = sf;
+                double* d = &L.slot_d[2 * s];
+                double* e = &L.slot_e[2 * s];
+                const double* F = &FACE[5 * (size_t)f];
+                if (f < nf) {
+                    const int own = ra.neighbour[2 * f], nei = ra.neighbour[2 * f + 1];
+                    const double xn = CELL[2 * nei] + FS[2 * f], yn = CELL[2 * nei + 1] + FS[2 * f + 1];
+                    if (sf > 0) {
+                        d[0] = F[0] - CELL[2 * own];
+                        d[1] = F[1] - CELL[2 * own + 1];
+                        e[0] = xn - CELL[2 * own];
+                        e[1] = yn - CELL[2 * own + 1];
+                    } else {
+                        d[0] = F[0] - xn;
+                        d[1] = F[1] - yn;
+                        e[0] = CELL[2 * own] - xn;
+                        e[1] = CELL[2 * own + 1] - yn;
+                    }
+                    const int side = sf < 0;
+                    const double* a = &ra.RBFbasis[(size_t)2 * (maxS + 1) * f + (size_t)side * (maxS + 1)];
+                    for (int i = 0; i <= ra.stencilSize[c]; ++i) L.rbf[L.sell(c, fi * (maxS + 1) + i, KR)] = a[i];
+                } else {
+                    const int b = f - nf;
+                    d[0] = F[0] - CELL[2 * c];
+                    d[1] = F[1] - CELL[2 * c + 1];
+                    e[0] = CELL[2 * ((size_t)nc + b)] - CELL[2 * c];
+                    e[1] = CELL[2 * ((size_t)nc + b) + 1] - CELL[2 * c + 1];
+                }
+            }
+        }
+    });
+
+    // internal faces
+    L.fcell.resize(2 * (size_t)nf);
+    L.fgeo.resize(8 * (size_t)nf);
+    parallel_for(nf, [&](int64_t f0, int64_t f1) {
+        for (int64_t f = f0; f < f1; ++f) {
+            const int own = ra.neighbour[2 * f], nei = ra.neighbour[2 * f + 1];
+            L.fcell[2 * f] = own;
+            L.fcell[2 * f + 1] = nei;
+            const double* F = &FACE[5 * f];
+            const double xn = CELL[2 * nei] + FS[2 * f], yn = CELL[2 * nei + 1] + FS[2 * f + 1];
+            const double ex = xn - CELL[2 * own], ey = yn - CELL[2 * own + 1];
+            double* g = &L.fgeo[8 * f];
+            g[0] = F[3];
+            g[1] = F[4];
+            g[2] = F[2];
+            g[3] = sqrt(ex * ex + ey * ey);
+            g[4] = F[0] - CELL[2 * own];
+            g[5] = F[1] - CELL[2 * own + 1];
+            g[6] = F[0] - xn;
+            g[7] = F[1] - yn;
+        }
+    });
+    // boundary faces
+    L.bcell = ra.boundaryFaceNeiLabel;
+    L.blabel = ra.boundaryFacesType;
+    L.bgeo.assign(4 * (size_t)nb, 0.0);
+    L.bval = ra.BVALUE;
+    if (L.bval.size() != 4 * (size_t)nb) L.bval.assign(4 * (size_t)nb, 0.0);
+    for (int b = 0; b < nb; ++b) {
+        const double* F = &FACE[5 * ((size_t)nf + b)];
+        L.bgeo[4 * b] = F[3];
+        L.bgeo[4 * b + 1] = F[4];
+        L.bgeo[4 * b + 2] = F[2];
+    }
+    return L;
+}
+
+}  // namespace afar

@@ -1,0 +1,403 @@
+// Device solver implementation: buffer management, the RK3 stage loop
+// (src/CUDA/cudaEuler2DmixO3RK3/GPUevolve.cu:177-306) and graph capture.
+#include "solver.h"
+
+#include <string.h>
+
+#include <cstdio>
+
+#include "kernels.h"
+
+namespace afar {
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(2, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t n) {
+        bytes = n;
+        check_hip(hipMalloc(&p, n ? n : 16), "hipMalloc");
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <typename R>
+class SolverImpl final : public DeviceSolver {
+    using v4 = typename VT<R>::v4;
+    using v2 = typename VT<R>::v2;
+
+   public:
+    SolverImpl(const RefArrays& ra, const SolverConfig& cfg) : ra_(ra) {
+        cfg_ = cfg;
+        L_ = make_dev_layout(ra, cfg.tile);
+        check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        check_hip(hipEventCreate(&ev0_), "hipEventCreate");
+        check_hip(hipEventCreate(&ev1_), "hipEventCreate");
+        lds_recon_ = (size_t)(L_.tile + L_.maxHalo) * sizeof(v4);
+        lds_bvd_ = (size_t)(L_.tile + L_.maxHalo);
+        int dev = 0;
+        check_hip(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t prop;
+        check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        size_t lds_cap = prop.sharedMemPerBlock;
+        if (lds_recon_ > lds_cap)
+            throw Error(3, "Too large local block stencil size: reconstruct needs " + std::to_string(lds_recon_) +
+                               " bytes of LDS, the device allows " + std::to_string(lds_cap));
+        check_hip(set_lds_limits<R>(lds_recon_ > 65536 ? lds_recon_ : 65536), "hipFuncSetAttribute");
+        upload();
+    }
+    ~SolverImpl() override {
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        if (stream_) {
+            (void)hipStreamSynchronize(stream_);
+            (void)hipStreamDestroy(stream_);
+        }
+        (void)hipEventDestroy(ev0_);
+        (void)hipEventDestroy(ev1_);
+    }
+
+    double compute_dt() override {
+        launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
+        check_hip(hipGetLastError(), "k_cfl");
+        double dt = 0.0;
+        check_hip(hipMemcpyAsync(&dt, P_.dt, sizeof(double), hipMemcpyDeviceToHost, stream_), "dt copy");
+        check_hip(hipStreamSynchronize(stream_), "dt sync");
+        return dt;
+    }
+
+    void evolve(double dt) override {
+        enqueue_stages(dt, nullptr);
+        check_hip(hipGetLastError(), "evolve");
+    }
+
+    void evolve_auto(int nsteps) override {
+        if (!graph_exec_) capture();
+        for (int i = 0; i < nsteps; ++i) check_hip(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+    }
+
+    void synchronize() override { check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+    double last_dt() override {
+        double dt = 0.0;
+        check_hip(hipMemcpyAsync(&dt, P_.dt, sizeof(double), hipMemcpyDeviceToHost, stream_), "dt copy");
+        synchronize();
+        return dt;
+    }
+
+    void copy_out(double* FD, int8_t* shock) override {
+        const int nc = L_.nc, nb = L_.nb;
+        std::vector<v4> w(nc + nb);
+        std::vector<int8_t> s(nc + nb), t(nc);
+        check_hip(hipMemcpyAsync(w.data(), P_.W, sizeof(v4) * (nc + nb), hipMemcpyDeviceToHost, stream_), "W copy");
+        check_hip(hipMemcpyAsync(s.data(), P_.smooth, nc + nb, hipMemcpyDeviceToHost, stream_), "flag copy");
+        check_hip(hipMemcpyAsync(t.data(), P_.ftype, nc, hipMemcpyDeviceToHost, stream_), "type copy");
+        synchronize();
+        if (FD)
+            for (int i = 0; i < nc + nb; ++i) {
+                FD[4 * i] = w[i].x;
+                FD[4 * i + 1] = w[i].y;
+                FD[4 * i + 2] = w[i].z;
+                FD[4 * i + 3] = w[i].w;
+            }
+        if (shock)
+            for (int i = 0; i < nc + nb; ++i) {
+                shock[2 * i] = s[i];
+                shock[2 * i + 1] = i < nc ? t[i] : 0;
+            }
+    }
+
+    void get_conserved(double* U) override {
+        check_hip(hipMemcpyAsync(U, P_.U, sizeof(double4) * L_.nc, hipMemcpyDeviceToHost, stream_), "U copy");
+        synchronize();
+    }
+
+    void set_conserved(const double* U) override {
+        // primitive state from the conserved one, as the update kernel does it
+        const double gamma = cfg_.R / cfg_.Cv + 1.0;
+        for (int i = 0; i < L_.nc; ++i) {
+            const double rho = U[4 * i], rhoUx = U[4 * i + 1], rhoUy = U[4 * i + 2], rhoE = U[4 * i + 3];
+            const double p = (gamma - 1.0) * (rhoE - 0.5 * (rhoUx * rhoUx + rhoUy * rhoUy) / rho);
+            ra_.FD[4 * i] = p;
+            ra_.FD[4 * i + 1] = rhoUx / rho;
+            ra_.FD[4 * i + 2] = rhoUy / rho;
+            ra_.FD[4 * i + 3] = p / (cfg_.R * rho);
+        }
+        apply_bc_host(ra_);
+        ra_.cFD.assign(U, U + 4 * (size_t)L_.nc);
+        upload_state();
+    }
+
+    double time_steps(int nsteps, bool graph) override {
+        if (graph && !graph_exec_) capture();
+        synchronize();
+        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+        if (graph) {
+            for (int i = 0; i < nsteps; ++i) check_hip(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+        } else {
+            for (int i = 0; i < nsteps; ++i) {
+                launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
+                enqueue_stages(0.0, P_.dt);
+            }
+        }
+        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
+        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
+        float ms = 0.f;
+        check_hip(hipEventElapsedTime(&ms, ev0_, ev1_), "hipEventElapsedTime");
+        return ms;
+    }
+
+    // Times every launch of `reps` auto-dt steps with events on the solver's stream.
+    void profile(double* ms, int* count, int reps) override {
+        std::vector<hipEvent_t> ev;
+        std::vector<int> kind;
+        auto mark = [&](int k) {
+            hipEvent_t e;
+            check_hip(hipEventCreate(&e), "hipEventCreate");
+            check_hip(hipEventRecord(e, stream_), "hipEventRecord");
+            ev.push_back(e);
+            kind.push_back(k);
+        };
+        synchronize();
+        mark(-1);
+        for (int r = 0; r < reps; ++r) {
+            launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
+            mark(kKernCfl);
+            for (int s = 0; s < 3; ++s) {
+                launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_);
+                mark(kKernRecon);
+                launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
+                mark(kKernBvd);
+                launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
+                mark(kKernFlux);
+                launch_update<R>(P_, cfg_.R, gamma(), 0.0, P_.dt, b1(s), b2(s), b3(s), s == 2, stream_);
+                mark(kKernUpdate);
+            }
+        }
+        check_hip(hipEventSynchronize(ev.back()), "hipEventSynchronize");
+        for (int k = 0; k < kKernKinds; ++k) {
+            ms[k] = 0.0;
+            count[k] = 0;
+        }
+        for (size_t i = 1; i < ev.size(); ++i) {
+            float t = 0.f;
+            check_hip(hipEventElapsedTime(&t, ev[i - 1], ev[i]), "hipEventElapsedTime");
+            ms[kind[i]] += t;
+            count[kind[i]] += 1;
+        }
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+
+    const DevLayout& layout() const override { return L_; }
+
+    void set_constants(double Rgas, double Cv, double CFL) override {
+        if (Rgas == cfg_.R && Cv == cfg_.Cv && CFL == cfg_.CFL) return;
+        cfg_.R = Rgas;
+        cfg_.Cv = Cv;
+        cfg_.CFL = CFL;
+        synchronize();
+        if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        graph_exec_ = nullptr;
+        graph_ = nullptr;
+    }
+
+    void debug_fetch(const char* what, void* out, size_t bytes) override {
+        const void* src = nullptr;
+        size_t have = 0;
+        auto pick = [&](const char* name, const void* p, size_t n) {
+            if (strcmp(what, name) == 0) {
+                src = p;
+                have = n;
+            }
+        };
+        const size_t nc = L_.nc, nf = L_.nf, nb = L_.nb;
+        pick("W", P_.W, sizeof(v4) * (nc + nb));
+        pick("U", P_.U, sizeof(double4) * nc);
+        pick("gp", P_.gp, sizeof(v2) * nc);
+        pick("gu", P_.gu, sizeof(v2) * nc);
+        pick("gv", P_.gv, sizeof(v2) * nc);
+        pick("gT", P_.gT, sizeof(v2) * nc);
+        pick("lim", P_.lim, sizeof(v4) * nc);
+        pick("smooth", P_.smooth, nc + nb);
+        pick("ftype", P_.ftype, nc);
+        pick("fv", P_.fv, sizeof(v4) * 2 * nf);
+        pick("flux", P_.flux, sizeof(double4) * (nf + nb));
+        pick("lsq", P_.lsq, sizeof(v2) * (size_t)L_.nwaves * L_.maxC * 64);
+        if (!src) throw Error(1, std::string("unknown device array ") + what);
+        if (bytes != have) throw Error(1, std::string("size mismatch fetching ") + what);
+        synchronize();
+        check_hip(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost), "debug copy");
+    }
+
+    void run_stage_kernels(int upto, double dt, int stage) override {
+        launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_);
+        if (upto > 0) launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
+        if (upto > 1) launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
+        if (upto > 2) launch_update<R>(P_, cfg_.R, gamma(), dt, nullptr, b1(stage), b2(stage), b3(stage), stage == 2, stream_);
+        check_hip(hipGetLastError(), "stage kernels");
+    }
+
+   private:
+    double gamma() const { return cfg_.R / cfg_.Cv + 1.0; }
+    // SSP-RK3 Shu-Osher coefficients: the reference's truncated literals
+    // (GPUevolve.cu:223-225) or the exact fractions.
+    double b1(int s) const {
+        static const double lit[3] = {1.0, 0.75, 0.333333}, ex[3] = {1.0, 0.75, 1.0 / 3.0};
+        return cfg_.exactRK ? ex[s] : lit[s];
+    }
+    double b2(int s) const {
+        static const double lit[3] = {0.0, 0.25, 0.666667}, ex[3] = {0.0, 0.25, 2.0 / 3.0};
+        return cfg_.exactRK ? ex[s] : lit[s];
+    }
+    double b3(int s) const {
+        static const double lit[3] = {1.0, 0.25, 0.666667}, ex[3] = {1.0, 0.25, 2.0 / 3.0};
+        return cfg_.exactRK ? ex[s] : lit[s];
+    }
+
+    void enqueue_stages(double dt, const double* dt_ptr) {
+        for (int s = 0; s < 3; ++s) {
+            launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_);
+            launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
+            launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
+            launch_update<R>(P_, cfg_.R, gamma(), dt, dt_ptr, b1(s), b2(s), b3(s), s == 2, stream_);
+        }
+    }
+
+    void capture() {
+        synchronize();
+        check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
+        enqueue_stages(0.0, P_.dt);
+        check_hip(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
+        check_hip(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+
+    template <typename T> T* put(DevMem& m, const T* src, size_t n) {
+        m.alloc(sizeof(T) * n);
+        if (n && src) check_hip(hipMemcpy(m.p, src, sizeof(T) * n, hipMemcpyHostToDevice), "upload");
+        return m.as<T>();
+    }
+
+    void upload_state() {
+        const int nc = L_.nc, nb = L_.nb;
+        std::vector<v4> w(nc + nb);
+        for (int i = 0; i < nc + nb; ++i) {
+            w[i].x = (R)ra_.FD[4 * i];
+            w[i].y = (R)ra_.FD[4 * i + 1];
+            w[i].z = (R)ra_.FD[4 * i + 2];
+            w[i].w = (R)ra_.FD[4 * i + 3];
+        }
+        synchronize();
+        check_hip(hipMemcpy(P_.W, w.data(), sizeof(v4) * (nc + nb), hipMemcpyHostToDevice), "W upload");
+        check_hip(hipMemcpy(P_.U, ra_.cFD.data(), sizeof(double4) * nc, hipMemcpyHostToDevice), "U upload");
+        check_hip(hipMemcpy(P_.Uold, ra_.cFD.data(), sizeof(double4) * nc, hipMemcpyHostToDevice), "Uold upload");
+    }
+
+    void upload() {
+        const size_t nc = L_.nc, nf = L_.nf, nb = L_.nb;
+        const size_t nslot = (size_t)L_.nwaves * L_.maxC * 64;
+        P_.nc = L_.nc;
+        P_.nf = L_.nf;
+        P_.nb = L_.nb;
+        P_.maxS = L_.maxS;
+        P_.maxC = L_.maxC;
+        P_.ntiles = L_.ntiles;
+        m_W.alloc(sizeof(v4) * (nc + nb));
+        P_.W = m_W.as<v4>();
+        m_U.alloc(sizeof(double4) * nc);
+        P_.U = m_U.as<double4>();
+        m_Uold.alloc(sizeof(double4) * nc);
+        P_.Uold = m_Uold.as<double4>();
+        P_.vol = put(m_vol, L_.vol.data(), nc);
+        for (DevMem* m : {&m_gp, &m_gu, &m_gv, &m_gT}) m->alloc(sizeof(v2) * nc);
+        P_.gp = m_gp.as<v2>();
+        P_.gu = m_gu.as<v2>();
+        P_.gv = m_gv.as<v2>();
+        P_.gT = m_gT.as<v2>();
+        m_lim.alloc(sizeof(v4) * nc);
+        P_.lim = m_lim.as<v4>();
+        m_smooth.alloc(nc + nb);
+        P_.smooth = m_smooth.as<int8_t>();
+        check_hip(hipMemset(P_.smooth, 0, nc + nb), "hipMemset");
+        m_ftype.alloc(nc);
+        P_.ftype = m_ftype.as<int8_t>();
+        check_hip(hipMemset(P_.ftype, 0, nc), "hipMemset");
+        P_.scnt = put(m_scnt, L_.scnt.data(), nc);
+        P_.ccnt = put(m_ccnt, L_.ccnt.data(), nc);
+        P_.sten = put(m_sten, L_.sten.data(), L_.sten.size());
+        P_.slot_face = put(m_slot_face, L_.slot_face.data(), nslot);
+        P_.slot_d = reinterpret_cast<const double2*>(put(m_slot_d, L_.slot_d.data(), 2 * nslot));
+        P_.slot_e = reinterpret_cast<const double2*>(put(m_slot_e, L_.slot_e.data(), 2 * nslot));
+        m_lsq.alloc(sizeof(v2) * nslot);
+        P_.lsq = m_lsq.as<v2>();
+        {
+            std::vector<R> rbf(L_.rbf.size());
+            for (size_t i = 0; i < rbf.size(); ++i) rbf[i] = (R)L_.rbf[i];
+            P_.rbf = put(m_rbf, rbf.data(), rbf.size());
+        }
+        P_.halo_ptr = put(m_halo_ptr, L_.halo_ptr.data(), L_.halo_ptr.size());
+        P_.halo = put(m_halo, L_.halo.data(), L_.halo.size());
+        P_.fcell = reinterpret_cast<const int2*>(put(m_fcell, L_.fcell.data(), 2 * nf));
+        P_.fgeo = reinterpret_cast<const double4*>(put(m_fgeo, L_.fgeo.data(), 8 * nf));
+        m_fv.alloc(sizeof(v4) * 2 * nf);
+        P_.fv = m_fv.as<v4>();
+        m_flux.alloc(sizeof(double4) * (nf + nb));
+        P_.flux = m_flux.as<double4>();
+        P_.bcell = put(m_bcell, L_.bcell.data(), nb);
+        P_.blabel = put(m_blabel, L_.blabel.data(), nb);
+        P_.bgeo = reinterpret_cast<const double4*>(put(m_bgeo, L_.bgeo.data(), 4 * nb));
+        P_.bval = reinterpret_cast<const double4*>(put(m_bval, L_.bval.data(), 4 * nb));
+        m_dt.alloc(sizeof(double) + sizeof(unsigned long long) + sizeof(unsigned int) + 16);
+        P_.dt = m_dt.as<double>();
+        P_.dt_accum = reinterpret_cast<unsigned long long*>(m_dt.as<char>() + 8);
+        P_.dt_count = reinterpret_cast<unsigned int*>(m_dt.as<char>() + 16);
+        {
+            const double init_dt = 1e5;
+            unsigned long long acc;
+            memcpy(&acc, &init_dt, 8);
+            unsigned int zero = 0;
+            check_hip(hipMemcpy(P_.dt, &init_dt, 8, hipMemcpyHostToDevice), "dt init");
+            check_hip(hipMemcpy(P_.dt_accum, &acc, 8, hipMemcpyHostToDevice), "dt init");
+            check_hip(hipMemcpy(P_.dt_count, &zero, 4, hipMemcpyHostToDevice), "dt init");
+        }
+        upload_state();
+        // K3: least-squares weights, once (GPUevolve.cu:131-140)
+        launch_lsq<R>(P_, stream_);
+        check_hip(hipGetLastError(), "k_lsq");
+        synchronize();
+    }
+
+    RefArrays ra_;
+    DevLayout L_;
+    DevPtrs<R> P_{};
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    size_t lds_recon_ = 0, lds_bvd_ = 0;
+    DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
+        m_slot_face, m_slot_d, m_slot_e, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fv, m_flux, m_bcell,
+        m_blabel, m_bgeo, m_bval, m_dt;
+};
+
+}  // namespace
+
+std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg) {
+    if (ra.nc <= 0) throw Error(1, "empty mesh");
+    if (cfg.precision == 1) return std::unique_ptr<DeviceSolver>(new SolverImpl<float>(ra, cfg));
+    return std::unique_ptr<DeviceSolver>(new SolverImpl<double>(ra, cfg));
+}
+
+}  // namespace afar

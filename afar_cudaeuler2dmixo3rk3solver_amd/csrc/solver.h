@@ -1,0 +1,52 @@
+// Device solver: owns the HIP buffers of one mesh, its stream and the
+// captured step graph, and drives the stage pipeline.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "layout.h"
+
+namespace afar {
+
+struct SolverConfig {
+    double R = 0.7142857, Cv = 1.7857142857, CFL = 0.5;
+    int precision = 0;  // 0 FP64, 1 FP32-mixed
+    int exactRK = 0;
+    int tile = 256;
+};
+
+// kernel kinds reported by profile()
+enum { kKernCfl = 0, kKernRecon = 1, kKernBvd = 2, kKernFlux = 3, kKernUpdate = 4, kKernKinds = 5 };
+
+class DeviceSolver {
+   public:
+    virtual ~DeviceSolver() = default;
+    virtual double compute_dt() = 0;                    // CFL pass, result copied to the host
+    virtual void evolve(double dt) = 0;                 // one SSP-RK3 step with a host time step
+    virtual void evolve_auto(int nsteps) = 0;           // CFL + step on the device, graph replay
+    virtual void synchronize() = 0;
+    virtual void copy_out(double* FD, int8_t* shock) = 0;  // FD 4*(nc+nb); shock 2*(nc+nb) interleaved
+    virtual void get_conserved(double* U) = 0;          // 4*nc
+    virtual void set_conserved(const double* U) = 0;    // also recomputes W and ghosts
+    virtual double last_dt() = 0;
+    virtual double time_steps(int nsteps, bool graph) = 0;  // hipEvent milliseconds for nsteps
+    virtual void profile(double* ms, int* count, int reps) = 0;
+    virtual const DevLayout& layout() const = 0;
+    virtual void debug_fetch(const char* what, void* out, size_t bytes) = 0;
+    // Runs reconstruct, then (upto >= 1) bvd, (>= 2) flux, (>= 3) update of RK stage `stage`.
+    virtual void run_stage_kernels(int upto, double dt, int stage) = 0;
+    const SolverConfig& config() const { return cfg_; }
+    // Gas constants and CFL number; a captured step graph is rebuilt on change.
+    virtual void set_constants(double R, double Cv, double CFL) = 0;
+
+   protected:
+    SolverConfig cfg_;
+};
+
+std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg);
+void check_hip(hipError_t e, const char* what);
+
+}  // namespace afar

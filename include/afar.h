@@ -1,0 +1,237 @@
+/*
+ * afar.h -- C-ABI of the MI355X-native AFAR SSP-RK3 pipeline.
+ *
+ * Two seams of the reference solver are reproduced:
+ *
+ *   Seam 1, application <-> solver object: the C++ class basicCUDAsolver
+ *   (src/CUDA/cudaEuler2DmixO3RK3/basicCUDAsolver.h:9-53) called from main()
+ *   (applications/solvers/compressible/CUDA/cudaEuler2DmixO3RK3solver/
+ *   cudaEuler2DmixO3RK3solver.cpp:46,58,64,68,85).  Here: afar_mesh_* builds
+ *   the mesh and its precompute, afar_solver_* is the solver object.
+ *
+ *   Seam 2, host <-> device: the free functions over fieldPointer of
+ *   src/CUDA/cudaEuler2DmixO3RK3/GPUevolve.h:6-59, called from
+ *   basicCUDAsolver.cpp:80-96,115,856,864-870,876.  Here: afar_set_device_*,
+ *   afar_free_field_data, afar_adjust_time_step, afar_gpu_evolve,
+ *   afar_copy_field_data_device_to_host over an opaque afar_device.
+ *
+ * Every function returns AFAR_OK (0) or an error code and never exits the
+ * process; afar_last_error() gives the message of the calling thread's last
+ * failure.  Handles are independent: calls on different handles may run on
+ * different host threads.  All sizes are the reference's 32-bit counts.
+ */
+#ifndef AFAR_H
+#define AFAR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AFAR_OK 0
+#define AFAR_ERR_INVALID 1 /* bad argument, mesh or dictionary value */
+#define AFAR_ERR_HIP 2     /* HIP runtime error */
+#define AFAR_ERR_LIMIT 3   /* stencil or LDS limit exceeded */
+#define AFAR_ERR_IO 4      /* basis cache file error */
+
+typedef struct afar_mesh afar_mesh;
+typedef struct afar_solver afar_solver;
+typedef struct afar_device afar_device;
+
+/* Dictionary values of the reference case:
+ *   fvSchemes CUDA{R; Cv; stencilType; recalcBasis}  (basicCUDAsolver.cpp:55-68)
+ *   fvSchemes shapeFactor (top level, default 10)     (basicCUDAsolver.cpp:58)
+ *   fvSchemes gaussQuadrature{cOrder 5; fOrder 4}     (d2GaussQuadrature.C:12-13)
+ *   controlDict maxCo                                 (basicCUDAsolver.cpp:57)
+ * plus this library's extensions: precision, exactRK, tile. */
+typedef struct afar_schemes {
+    double R;
+    double Cv;
+    double maxCo;
+    double shapeFactor;
+    int32_t stencilType; /* 0 faceNeighbour, 1 edgeNeighbour */
+    int32_t recalcBasis; /* 1 compute the RBF basis, 0 read system/basis{Own,Nei}List */
+    int32_t cOrder;
+    int32_t fOrder;
+    int32_t precision;   /* 0 all-FP64, 1 FP32 reconstruction/limiter/flux with FP64 state */
+    int32_t exactRK;     /* 0 the reference's RK3 literals (GPUevolve.cu:223-225), 1 exact 1/3, 2/3 */
+    int32_t tile;        /* cells per LDS tile / workgroup: 64, 128 or 256 (0 = 256) */
+} afar_schemes;
+
+const char* afar_last_error(void);
+const char* afar_version(void);
+/* Fills the reference defaults (R, Cv of the cornerFlow case, shapeFactor 10,
+ * edgeNeighbour, recalcBasis true, cOrder 5, fOrder 4, maxCo 0.5). */
+void afar_schemes_default(afar_schemes* s);
+
+/* ------------------------------------------------------------------ meshes
+ * Replaces the OpenFOAM fvMesh the reference reads (basicCUDAsolver.cpp:
+ * 122-258) and the offline `renumberMesh` (README.md:76).  Polygons are
+ * counter-clockwise in the x-y plane, extruded by `depth`.  Boundary edges
+ * carry a patch index; patch_labels are the CUDA.boundary.patch<i> labels
+ * 0..5 of the reference (src/CUDA/BCs/2D/cudaEulerBCs.h:34-39) or 6, this
+ * library's fixed value taken from patch_values (p, Ux, Uy, T per patch).
+ * cell_vshift (optional) gives per cell vertex the multiple of the period
+ * (period_x, period_y) added to the point, for periodic meshes. */
+int afar_mesh_from_polygons(int64_t npoints, const double* xy, int64_t ncells, const int64_t* cell_ptr,
+                            const int32_t* cell_pts, const int8_t* cell_vshift, double period_x, double period_y,
+                            int64_t nbedges, const int32_t* bedge_pts, const int32_t* bedge_patch, int32_t npatches,
+                            const int32_t* patch_labels, const double* patch_values, double depth, int32_t renumber,
+                            afar_mesh** out);
+/* Supersonic corner: nx x ny quads on [0,lx] x [wall(x), ly], wall bending
+ * down by angle_deg at corner_x; interior nodes jittered by +-jitter spacing.
+ * Patches: 0 inlet (label 6, inflow p,Ux,Uy,T), 1 outlet (4), 2 lower wall (2), 3 upper (top_label). */
+int afar_mesh_generate_corner(int32_t nx, int32_t ny, double lx, double ly, double jitter, uint64_t seed,
+                              double corner_x, double angle_deg, const double* inflow, int32_t top_label,
+                              afar_mesh** out);
+/* Unit square, n x n quads, interior nodes jittered, each quad split into two
+ * triangles with probability split_p (diagonal chosen by a second draw).
+ * Four patches (left, right, bottom, top), all label 4. */
+int afar_mesh_generate_riemann(int32_t n, double jitter, uint64_t seed, double split_p, afar_mesh** out);
+/* Doubly periodic [0,L]^2, n x n quads split into 2n^2 triangles, all nodes jittered. */
+int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t seed, afar_mesh** out);
+void afar_mesh_destroy(afar_mesh* m);
+/* out[0..7] = nc, nf, nb, npatches, maxS, maxC, maxLocalBlockStencilSize,
+ * maxCompactLocalBlockStencilSize (the last four are 0 before precompute;
+ * the block sizes are those of the reference's 128-cell tables,
+ * basicCUDAsolver.cpp:697-850). */
+int afar_mesh_sizes(const afar_mesh* m, int64_t* out);
+/* Cell and pseudo-cell centres 2*(nc+nb), volumes nc (mesh order). */
+int afar_mesh_cells(const afar_mesh* m, double* centres, double* volumes);
+/* perm[i] = caller's index of mesh cell i (the RCM order). */
+int afar_mesh_permutation(const afar_mesh* m, int32_t* perm);
+/* own nf+nb, nei nf, face 5*(nf+nb) (x, y, magSf, Sfx, Sfy), fshift 2*nf, patch nb. */
+int afar_mesh_faces(const afar_mesh* m, int32_t* own, int32_t* nei, double* face, double* fshift, int32_t* bpatch);
+/* Cell polygons in mesh order: vptr nc+1, vxy 2*vptr[nc] (coordinates in the cell's frame). */
+int afar_mesh_polygons(const afar_mesh* m, int32_t* vptr, double* vxy);
+
+/* ------------------------------------------------------------------ precompute
+ * Stencils, Gauss quadrature, RBF moment matrices, pseudo-inverses and face
+ * coefficients of basicCUDAsolver::initStencilData (basicCUDAsolver.cpp:
+ * 261-694).  basis_dir: when recalcBasis is 1 and basis_dir is not NULL the
+ * coefficients are written there as basisOwnList/basisNeiList; when
+ * recalcBasis is 0 they are read from there (basicCUDAsolver.cpp:626-694). */
+int afar_mesh_precompute(afar_mesh* m, const afar_schemes* s, const char* basis_dir);
+/* S nc, C nc, stencil maxS*nc (global ids, pseudo-cells >= nc), cellFaces maxC*nc (+-(face+1)). */
+int afar_mesh_stencils(const afar_mesh* m, uint8_t* S, uint8_t* C, int32_t* stencil, int32_t* cellFaces);
+/* rbf 2*(maxS+1)*nf (RBFbasis layout), trouble nc, refDistance nc; NULL to skip. */
+int afar_mesh_rbf_basis(const afar_mesh* m, double* rbf, uint8_t* trouble, double* refDistance);
+/* Gauss points of a stencil member (cell < nc: cell rule cOrder; pseudo-cell: face rule fOrder). */
+int afar_mesh_gauss_points(const afar_mesh* m, int32_t member, int32_t order, double* xy, double* w, int32_t cap,
+                           int32_t* count);
+int afar_mesh_write_basis(const afar_mesh* m, const char* dir);
+/* Moore-Penrose pseudo-inverse (one-sided Jacobi SVD), row-major n x n; the
+ * precompute's replacement of Foam::SVDinv (basicCUDAsolver.cpp:509). */
+int afar_pinv(int32_t n, const double* A, double* G);
+
+/* ------------------------------------------------------------------ seam 1: solver object */
+/* basicCUDAsolver(runTime, mesh) (basicCUDAsolver.cpp:9-110): W0 = p, Ux, Uy, T
+ * per cell in the caller's order (the 0/ fields); ghost states come from the
+ * boundary table.  The mesh must have been precomputed with the same schemes. */
+int afar_solver_create(const afar_mesh* m, const afar_schemes* s, const double* W0, afar_solver** out);
+/* basicCUDAsolver::setDeltaT (basicCUDAsolver.cpp:854-859): global CFL time step. */
+int afar_solver_set_delta_t(afar_solver* h, double* dt);
+/* basicCUDAsolver::evolve (basicCUDAsolver.cpp:862-871): one SSP-RK3 step of size dt.
+ * Asynchronous: returns once the work is queued on the solver's stream. */
+int afar_solver_evolve(afar_solver* h, double dt);
+/* nsteps of (CFL time step + RK3 step) with the time step kept on the
+ * device; one captured hipGraph per step.  Asynchronous. */
+int afar_solver_evolve_auto(afar_solver* h, int32_t nsteps);
+/* Time step used by the last evolve_auto step (synchronises). */
+int afar_solver_last_dt(afar_solver* h, double* dt);
+int afar_solver_synchronize(afar_solver* h);
+/* basicCUDAsolver::writeField (basicCUDAsolver.cpp:874-913): W 4*nc and
+ * shock nc (final reconstruction type, shockIndicator[2c+1]) in the caller's
+ * order, ghost 4*nb in boundary-face order.  NULL pointers are skipped. */
+int afar_solver_write_field(afar_solver* h, double* W, double* ghost, int8_t* shock);
+/* Conserved state rho, rhoUx, rhoUy, rhoE (4*nc, caller's order): exact
+ * checkpoint / restart of the FP64 state (the reference rebuilds it from p, U, T). */
+int afar_solver_get_conserved(afar_solver* h, double* U);
+int afar_solver_set_conserved(afar_solver* h, const double* U);
+/* hipEvent time of nsteps auto-dt steps on the solver's stream (graph replay if use_graph). */
+int afar_solver_time_steps(afar_solver* h, int32_t nsteps, int32_t use_graph, double* ms);
+/* Per-kernel event times of reps auto-dt steps: ms[5], count[5] for
+ * {cfl, reconstruct, bvd, flux, update}. */
+int afar_solver_profile(afar_solver* h, int32_t reps, double* ms, int32_t* count);
+/* Device sizes: out[0..7] = nc, nf, nb, maxS, maxC, tile, ntiles, maxHalo. */
+int afar_solver_sizes(afar_solver* h, int64_t* out);
+/* Test hooks: run stage kernels (0 reconstruct, 1 +bvd, 2 +flux, 3 +update of
+ * RK stage `stage` with step dt) and copy named device arrays (mesh order). */
+int afar_solver_run_stage(afar_solver* h, int32_t upto, double dt, int32_t stage);
+int afar_solver_fetch(afar_solver* h, const char* name, void* out, size_t bytes);
+void afar_solver_destroy(afar_solver* h);
+
+/* Reference-layout arrays of a precomputed mesh (what seam 2 receives), for
+ * checking against a host implementation.  The pointers in `out` must be
+ * allocated by the caller with the sizes listed at afar_field_pointer. */
+typedef struct afar_field_pointer afar_field_pointer;
+int afar_mesh_export_reference(const afar_mesh* m, const afar_schemes* s, const double* W0, afar_field_pointer* out);
+
+/* ------------------------------------------------------------------ seam 2: GPUevolve.h
+ * Host arrays of fieldPointer (src/CUDA/cudaEuler2DmixO3RK3/basicDataStructure.h:
+ * 75-160), same meaning and layout.  Device-only members (gradFD, Flux, Res,
+ * faceFD, matrix, minDeltaT) are internal.  The extension members may be NULL. */
+struct afar_field_pointer {
+    double* CELL;                      /* 2*(nc+nb): x, y (cells, then boundary-face centres) */
+    double* FACE;                      /* 5*(nf+nb): x, y, magSf, Sfx, Sfy */
+    double* cellVolume;                /* nc */
+    double* FD;                        /* 4*(nc+nb): p, Ux, Uy, T */
+    int8_t* shockIndicator;            /* 2*(nc+nb) */
+    double* cFD;                       /* 4*nc: rho, rhoUx, rhoUy, rhoE */
+    int32_t* neighbour;                /* 2*nf: own, nei */
+    int32_t* boundaryFacesNum;         /* npatches */
+    int32_t* boundaryFaceNeiLabel;     /* nb */
+    uint8_t* boundaryFacesType;        /* nb */
+    uint8_t* stencilSize;              /* nc */
+    uint8_t* compactStencilSize;       /* nc */
+    float* RBFbasis;                   /* 2*(maxS+1)*nf */
+    int32_t* extendStencil;            /* (maxLB-128)*ceil(nc/128) */
+    uint16_t* extendStencilSize;       /* ceil(nc/128) */
+    uint16_t* compactExtendStencilSize;/* ceil(nc/128) */
+    uint16_t* localStencil;            /* maxS*nc, 128-cell block-local ids */
+    int32_t* cellFaces;                /* maxC*nc */
+    /* extensions */
+    int32_t* globalStencil;            /* maxS*nc global ids; replaces the block tables when set */
+    double* RBFbasis64;                /* 2*(maxS+1)*nf FP64 coefficients; replaces RBFbasis when set */
+    double* faceShift;                 /* 2*nf periodic shift of the neighbour centre */
+    double* boundaryValue;             /* 4*nb fixed values of label 6 */
+};
+
+/* setDeviceFieldData #1 (GPUevolve.cu:13-64): geometry and state. */
+int afar_set_device_field_data(afar_device** dev, const afar_field_pointer* host, int32_t cellsNum,
+                               int32_t facesNum, int32_t totalBoundaryFacesNum);
+/* Optional, between the two uploads: precision (0 FP64, 1 FP32-mixed), tile, R, Cv. */
+int afar_device_configure(afar_device* dev, int32_t precision, int32_t tile);
+/* setDeviceFieldData #2 (GPUevolve.cu:67-148): connectivity, stencils,
+ * coefficients; computes the least-squares weights on the device (K3). */
+int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* host, int32_t cellsNum,
+                                 int32_t facesNum, int32_t patchesNum, int32_t totalBoundaryFacesNum,
+                                 int32_t maxStencilSize, int32_t maxCompactStencilSize,
+                                 int32_t maxLocalBlockStencilSize, int32_t maxCompactLocalBlockStencilSize);
+/* freeFieldData (GPUevolve.cu:151-174): releases the device and the handle. */
+int afar_free_field_data(afar_device* dev);
+/* adjustTimeStep (GPUevolve.cu:177-204). */
+int afar_adjust_time_step(afar_device* dev, int32_t facesNum, double R, double Cv, double CFL, double* deltaT);
+/* GPUevolve (GPUevolve.cu:207-306): one SSP-RK3 step; synchronous like the reference. */
+int afar_gpu_evolve(afar_device* dev, double R, double Cv, double deltaT, int32_t cellsNum, int32_t facesNum,
+                    int32_t totalBoundaryFacesNum, int32_t maxStencilSize, int32_t maxCompactStencilSize,
+                    int32_t maxLocalBlockStencilSize, int32_t maxCompactLocalBlockStencilSize);
+/* copyFieldDataDeviceToHost (GPUevolve.cu:309-318): FD and shockIndicator. */
+int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* dev, int32_t cellsNum,
+                                        int32_t totalBoundaryFacesNum);
+
+/* ------------------------------------------------------------------ diagnostics */
+/* Evaluates exp, erfc, |x|^(1/2) and 1/x of the FP64 flux path on the device: out 4*n. */
+int afar_selftest_math(const double* x, double* out, int32_t n);
+/* Same functions on the host. */
+int afar_host_math(const double* x, double* out, int32_t n);
+/* Name and compute units of the current device. */
+int afar_device_info(char* name, int32_t cap, int32_t* cus, int64_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AFAR_H */
