@@ -56,7 +56,7 @@ def lib():
         L.oracle_store_conserved.argtypes = [pf]
         L.oracle_evolve.argtypes = [pf, d, d, d, C.c_int]
         L.oracle_run.argtypes = [pf, d, d, d, C.c_int, C.c_int, P(d)]
-        L.oracle_roe_flux.argtypes = [P(d)] + [d] * 16
+        L.oracle_roe_flux.argtypes = [P(d)] + [d] * 17
         L.oracle_kfvs_flux.argtypes = [P(d)] + [d] * 15
         L.oracle_central_flux.argtypes = [P(d)] + [d] * 13
         L.oracle_exp.argtypes = [d]

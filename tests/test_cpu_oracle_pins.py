@@ -1,0 +1,147 @@
+"""Pins of the oracle's point functions against what mathematics fixes.
+
+- exp / erfc against mpmath (40 digits): <= 1 ulp and <= 8 ulp.
+- Roe and KFVS flux consistency: F(U, U) equals the physical Euler flux.
+- Roe upwinding: for a supersonic face with all wave speeds positive and no
+  entropy fix active, Roe(UL, UR) = F(UL) when the true jumps are used.
+- KFVS at high Mach number tends to the upwind flux.
+- The SSP-RK3 update kernel with exact coefficients reproduces the RK3
+  amplification polynomial 1 + z + z^2/2 + z^3/6 on y' = lambda y, and the
+  reference's truncated literals deviate from it by O(1e-6 z^3).
+"""
+import ctypes as C
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+R_GAS = 0.7142857
+CV = 1.7857142857
+GAMMA = (CV + R_GAS) / CV
+
+
+def ulp_err(a, b):
+    if a == b:
+        return 0.0
+    return abs(a - b) / np.spacing(abs(b))
+
+
+def test_exp_against_mpmath():
+    mp.mp.dps = 40
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-700, 700, 2000), rng.uniform(-2, 2, 2000), [0.0, 1.0, -1.0, 709.7, -700.0]])
+    worst = max(ulp_err(O.lib().oracle_exp(float(x)), float(mp.exp(mp.mpf(float(x))))) for x in xs)
+    assert worst <= 1.0
+    assert O.lib().oracle_exp(0.0) == 1.0
+    assert O.lib().oracle_exp(800.0) == float("inf")
+    assert O.lib().oracle_exp(-800.0) == 0.0
+
+
+def test_erfc_against_mpmath():
+    mp.mp.dps = 40
+    rng = np.random.default_rng(2)
+    xs = np.concatenate([rng.uniform(-6, 6, 3000), rng.uniform(0, 26, 1000), [0.0, 0.5, -0.5, 3.75]])
+    worst = 0.0
+    for x in xs:
+        ref = float(mp.erfc(mp.mpf(float(x))))
+        if ref > 1e-300:
+            worst = max(worst, ulp_err(O.lib().oracle_erfc(float(x)), ref))
+    assert worst <= 8.0
+    assert ulp_err(O.lib().oracle_erfc(0.0), 1.0) <= 1.0
+    assert O.lib().oracle_erfc(30.0) == 0.0
+    assert O.lib().oracle_erfc(-30.0) == 2.0
+
+
+def physical_flux(rho, p, ux, uy, T, Sfx, Sfy):
+    vn = ux * Sfx + uy * Sfy
+    E = CV * T + 0.5 * (ux * ux + uy * uy)
+    return np.array([rho * vn, rho * ux * vn + p * Sfx, rho * uy * vn + p * Sfy, rho * vn * (E + p / rho)])
+
+
+STATES = [(1.0, 1.0, 0.3, -0.2), (0.5323, 0.3, 1.206, 0.0), (0.138, 0.029, 1.206, 1.206), (1.5, 1.5, -2.0, 0.7)]
+
+
+@pytest.mark.parametrize("state", STATES)
+@pytest.mark.parametrize("normal", [(1.0, 0.0), (0.6, -0.8), (-0.3, 0.95)])
+def test_flux_consistency(state, normal):
+    rho, p, ux, uy = state
+    T = p / (rho * R_GAS)
+    mag = 0.37
+    Sfx, Sfy = normal[0] * mag / np.hypot(*normal), normal[1] * mag / np.hypot(*normal)
+    F = physical_flux(rho, p, ux, uy, T, Sfx, Sfy)
+    cen = O.central_flux(rho, rho, p, p, ux, ux, uy, uy, T, T, CV, Sfx, Sfy)
+    assert np.allclose(cen, F, rtol=1e-14, atol=1e-14)
+    roe = O.roe_flux(rho, rho, p, p, ux, ux, uy, uy, T, T, 0.0, 0.0, R_GAS, CV, mag, Sfx, Sfy, base=cen)
+    assert np.allclose(roe, F, rtol=1e-13, atol=1e-13)
+    kf = O.kfvs_flux(rho, rho, p, p, ux, ux, uy, uy, T, T, R_GAS, CV, mag, Sfx, Sfy, base=cen)
+    assert np.allclose(kf, F, rtol=1e-12, atol=1e-12)
+
+
+def test_roe_upwind_supersonic():
+    # left state moving right at Mach ~3; right state slower (no entropy-fix band: eps = 0)
+    rhoL, pL, uL, vL = 1.0, 1.0, 3.6, 0.1
+    rhoR, pR, uR, vR = 1.1, 1.05, 3.4, 0.05
+    TL, TR = pL / (rhoL * R_GAS), pR / (rhoR * R_GAS)
+    cL, cR = np.sqrt(GAMMA * pL / rhoL), np.sqrt(GAMMA * pR / rhoR)
+    assert uR - cR <= uL - cL and uR <= uL and uR + cR <= uL + cL and uR - cR > 0
+    # all wave speeds positive and no entropy fix: the Roe dissipation term is
+    # -1/2 sum lambda_k alpha_k r_k = -1/2 (F(UR) - F(UL)) (Roe's property A dU = dF)
+    for Sfx, Sfy in ((0.25, 0.0), (0.24, 0.07)):
+        mag = np.hypot(Sfx, Sfy)
+        FL = physical_flux(rhoL, pL, uL, vL, TL, Sfx, Sfy)
+        FR = physical_flux(rhoR, pR, uR, vR, TR, Sfx, Sfy)
+        diss = O.roe_flux(rhoL, rhoR, pL, pR, uL, uR, vL, vR, TL, TR, pR - pL, rhoR - rhoL, R_GAS, CV, mag, Sfx, Sfy)
+        assert np.allclose(diss, -0.5 * (FR - FL), rtol=1e-11, atol=1e-13)
+        # and the full flux is exactly upwind for the continuity equation
+        cen = O.central_flux(rhoL, rhoR, pL, pR, uL, uR, vL, vR, TL, TR, CV, Sfx, Sfy)
+        assert abs((cen + diss)[0] - FL[0]) < 1e-13
+        # a plausible slip (a jump taken with the wrong sign) breaks the property
+        bad = O.roe_flux(rhoL, rhoR, pL, pR, uL, uR, vL, vR, TL, TR, pL - pR, rhoR - rhoL, R_GAS, CV, mag, Sfx, Sfy)
+        assert not np.allclose(bad, -0.5 * (FR - FL), rtol=1e-6)
+
+
+def test_kfvs_hypersonic_upwind():
+    rhoL, pL, uL, vL = 1.0, 0.01, 20.0, 0.0
+    rhoR, pR, uR, vR = 1.0, 0.01, 19.9, 0.0
+    TL, TR = pL / (rhoL * R_GAS), pR / (rhoR * R_GAS)
+    Sfx, Sfy, mag = 1.0, 0.0, 1.0
+    cen = O.central_flux(rhoL, rhoR, pL, pR, uL, uR, vL, vR, TL, TR, CV, Sfx, Sfy)
+    kf = O.kfvs_flux(rhoL, rhoR, pL, pR, uL, uR, vL, vR, TL, TR, R_GAS, CV, mag, Sfx, Sfy, base=cen)
+    assert np.allclose(kf, physical_flux(rhoL, pL, uL, vL, TL, Sfx, Sfy), rtol=1e-10)
+
+
+def _one_cell_update(y0, lam, dt, betas):
+    """Drive the oracle's update kernel on a one-cell field with Res = -lam*y*V."""
+    lib = O.lib()
+    f = O.OracleFields()
+    f.nc, f.nf, f.nb, f.maxS, f.maxC = 1, 0, 0, 1, 1
+    V = np.array([2.0])
+    U = np.array([[y0, 0.0, 0.0, 10.0 * y0]])
+    Uold = U.copy()
+    Res = np.zeros((1, 4))
+    FD = np.zeros((1, 4))
+    f.cellVolume = O._p(V)
+    f.cFD = O._p(U)
+    f.cFDold = O._p(Uold)
+    f.Res = O._p(Res)
+    f.FD = O._p(FD)
+    for b1, b2, b3 in zip(*betas):
+        Res[:] = -lam * U * V[0]
+        lib.oracle_update(C.byref(f), R_GAS, R_GAS / CV + 1.0, dt, b1, b2, b3)
+    return U[0, 0]
+
+
+def test_rk3_amplification_polynomial():
+    exact = ((1.0, 0.75, 1.0 / 3.0), (0.0, 0.25, 2.0 / 3.0), (1.0, 0.25, 2.0 / 3.0))
+    literal = O.OracleState.RK_LITERAL
+    for z in (-0.1, -0.5, -1.0, -2.0, 0.3):
+        poly = 1 + z + z * z / 2 + z ** 3 / 6
+        assert abs(_one_cell_update(1.0, z, 1.0, exact) - poly) < 1e-14
+        # literal coefficients: the same form with 0.333333 / 0.666667
+        lit = _one_cell_update(1.0, z, 1.0, literal)
+        y1 = 1 + z
+        y2 = 0.75 + 0.25 * y1 + 0.25 * z * y1
+        assert abs(lit - (0.333333 + 0.666667 * y2 + 0.666667 * z * y2)) < 1e-14
+        assert abs(lit - poly) > 1e-9 * abs(z) ** 2 or z == 0
