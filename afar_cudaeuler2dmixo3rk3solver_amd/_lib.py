@@ -85,6 +85,7 @@ SIGNATURES = {
     "afar_mesh_generate_riemann": ([C.c_int32, C.c_double, C.c_uint64, C.c_double, P(_VP)], C.c_int),
     "afar_mesh_generate_periodic": ([C.c_int32, C.c_double, C.c_double, C.c_uint64, P(_VP)], C.c_int),
     "afar_mesh_destroy": ([_VP], None),
+    "afar_mesh_set_patch": ([_VP, C.c_int32, C.c_int32, _D], C.c_int),
     "afar_mesh_sizes": ([_VP, _I64], C.c_int),
     "afar_mesh_cells": ([_VP, _D, _D], C.c_int),
     "afar_mesh_permutation": ([_VP, _I32], C.c_int),

@@ -53,7 +53,25 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stdout + r.stderr))
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER = os.path.join(HERE, "cudaEuler2DmixO3RK3solver")
+HOST_SOURCES = ["foamDict.cpp", "basicCUDAsolver.cpp", "cudaEuler2DmixO3RK3solver.cpp"]
+
+
+def build_driver():
+    """The C++ application over the C-ABI (g++, linked against libafar.so)."""
+    srcs = [os.path.join(HERE, "host", s) for s in HOST_SOURCES]
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "host"), "-I",
+           os.path.join(HERE, "..", "include")] + srcs + ["-L", HERE, "-lafar", "-Wl,-rpath,$ORIGIN", "-o",
+                                                         DRIVER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed: %s\n%s" % (" ".join(cmd), r.stdout + r.stderr))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

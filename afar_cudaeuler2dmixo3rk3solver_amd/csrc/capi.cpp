@@ -172,6 +172,28 @@ int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t see
 
 void afar_mesh_destroy(afar_mesh* m) { delete m; }
 
+int afar_mesh_set_patch(afar_mesh* h, int32_t patch, int32_t label, const double* value) {
+    return guard([&] {
+        need(h != nullptr, "NULL mesh");
+        need(label >= 0 && label <= kMaxBCLabel, "boundary label out of range 0..6");
+        Mesh& m = h->m;
+        // patch counts over non-empty patches, as the reference's patch loop does
+        int idx = -1;
+        for (size_t p = 0; p < m.patches.size(); ++p) {
+            if (m.patch_size[p] == 0) continue;
+            if (++idx == patch) {
+                m.patches[p].label = label;
+                if (value)
+                    for (int k = 0; k < 4; ++k) m.patches[p].value[k] = value[k];
+                for (int b = m.patch_start[p]; b < m.patch_start[p] + m.patch_size[p]; ++b)
+                    m.bf_label[b] = (uint8_t)label;
+                return;
+            }
+        }
+        throw Error(AFAR_ERR_INVALID, "patch index out of range");
+    });
+}
+
 int afar_mesh_sizes(const afar_mesh* h, int64_t* out) {
     return guard([&] {
         need(h && out, "NULL argument");

@@ -93,6 +93,10 @@ int afar_mesh_generate_riemann(int32_t n, double jitter, uint64_t seed, double s
 /* Doubly periodic [0,L]^2, n x n quads split into 2n^2 triangles, all nodes jittered. */
 int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t seed, afar_mesh** out);
 void afar_mesh_destroy(afar_mesh* m);
+/* Sets the boundary label of the patch-th non-empty patch, as the reference
+ * reads CUDA.boundary.patch<i> (basicCUDAsolver.cpp:170-172); value (p, Ux,
+ * Uy, T) is used by label 6 and may be NULL to keep the current one. */
+int afar_mesh_set_patch(afar_mesh* m, int32_t patch, int32_t label, const double* value);
 /* out[0..7] = nc, nf, nb, npatches, maxS, maxC, maxLocalBlockStencilSize,
  * maxCompactLocalBlockStencilSize (the last four are 0 before precompute;
  * the block sizes are those of the reference's 128-cell tables,

@@ -198,14 +198,14 @@ void oracle_roe_flux(double* flux4, double rhoLeft, double rhoRight, double pLef
     const double contrVRight = UxRight * nfx + UyRight * nfy;
     const double hLeft = Cv * TLeft + 0.5 * (UxLeft * UxLeft + UyLeft * UyLeft) + pLeft / rhoLeft;
     const double hRight = Cv * TRight + 0.5 * (UxRight * UxRight + UyRight * UyRight) + pRight / rhoRight;
-    const double rhoTilde = sqrt(std::max(rhoLeft * rhoRight, 1e-20));
-    const double wLeft = sqrt(std::max(rhoLeft, 1e-20)) / (sqrt(std::max(rhoLeft, 1e-20)) + sqrt(std::max(rhoRight, 1e-20)));
+    const double rhoTilde = sqrt(fmax(rhoLeft * rhoRight, 1e-20));
+    const double wLeft = sqrt(fmax(rhoLeft, 1e-20)) / (sqrt(fmax(rhoLeft, 1e-20)) + sqrt(fmax(rhoRight, 1e-20)));
     const double wRight = 1.0 - wLeft;
     const double UxTilde = UxLeft * wLeft + UxRight * wRight;
     const double UyTilde = UyLeft * wLeft + UyRight * wRight;
     const double hTilde = hLeft * wLeft + hRight * wRight;
     const double qTildeSquare = UxTilde * UxTilde + UyTilde * UyTilde;
-    const double cTilde = sqrt(std::max((gamma - 1.0) * (hTilde - 0.5 * qTildeSquare), 1e-20));
+    const double cTilde = sqrt(fmax((gamma - 1.0) * (hTilde - 0.5 * qTildeSquare), 1e-20));
     const double contrVTilde = UxTilde * nfx + UyTilde * nfy;
     const double deltaUx = UxRight - UxLeft;
     const double deltaUy = UyRight - UyLeft;
@@ -215,15 +215,15 @@ void oracle_roe_flux(double* flux4, double rhoLeft, double rhoRight, double pLef
     const double r3 = (deltaP + rhoTilde * cTilde * deltaContrV) / (2.0 * cTilde * cTilde);
     const double UL = contrVLeft;
     const double UR = contrVRight;
-    const double cLeft = sqrt(std::max((gamma - 1.0) * (hLeft - 0.5 * (UxLeft * UxLeft + UyLeft * UyLeft)), 1e-20));
-    const double cRight = sqrt(std::max((gamma - 1.0) * (hRight - 0.5 * (UxRight * UxRight + UyRight * UyRight)), 1e-20));
-    double eps = 2.0 * std::max(0.0, (UR - cRight) - (UL - cLeft));
+    const double cLeft = sqrt(fmax((gamma - 1.0) * (hLeft - 0.5 * (UxLeft * UxLeft + UyLeft * UyLeft)), 1e-20));
+    const double cRight = sqrt(fmax((gamma - 1.0) * (hRight - 0.5 * (UxRight * UxRight + UyRight * UyRight)), 1e-20));
+    double eps = 2.0 * fmax(0.0,(UR - cRight) - (UL - cLeft));
     double lambda1 = fabs(contrVTilde - cTilde);
     lambda1 = (lambda1 >= eps) * lambda1 + (lambda1 < eps) * ((lambda1 * lambda1 + eps * eps) / (2.0 * eps + 1e-30));
-    eps = 2.0 * std::max(0.0, UR - UL);
+    eps = 2.0 * fmax(0.0,UR - UL);
     double lambda2 = fabs(contrVTilde);
     lambda2 = (lambda2 >= eps) * lambda2 + (lambda2 < eps) * ((lambda2 * lambda2 + eps * eps) / (2.0 * eps + 1e-30));
-    eps = 2.0 * std::max(0.0, (UR + cRight) - (UL + cLeft));
+    eps = 2.0 * fmax(0.0,(UR + cRight) - (UL + cLeft));
     double lambda3 = fabs(contrVTilde + cTilde);
     lambda3 = (lambda3 >= eps) * lambda3 + (lambda3 < eps) * ((lambda3 * lambda3 + eps * eps) / (2.0 * eps + 1e-30));
     rhoFlux -= 0.5 * (lambda1 * r1 + lambda2 * r2 + lambda3 * r3) * magSf;
