@@ -31,6 +31,7 @@ class Schemes(C.Structure):
         ("precision", C.c_int32),
         ("exactRK", C.c_int32),
         ("tile", C.c_int32),
+        ("boundaryFirstOrder", C.c_int32),
     ]
 
 

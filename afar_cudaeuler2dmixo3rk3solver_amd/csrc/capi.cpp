@@ -436,6 +436,7 @@ int afar_solver_create(const afar_mesh* h, const afar_schemes* s, const double* 
         cfg.precision = s->precision;
         cfg.exactRK = s->exactRK;
         cfg.tile = tile_of(s);
+        cfg.boundaryFirstOrder = s->boundaryFirstOrder != 0;
         auto sol = std::make_unique<afar_solver>();
         sol->dev = make_device_solver(ra, cfg);
         sol->perm = h->m.perm;

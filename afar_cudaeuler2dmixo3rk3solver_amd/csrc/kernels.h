@@ -7,6 +7,8 @@
 
 namespace afar {
 
+constexpr int kCflBlocks = 1024;
+
 template <typename R> struct VT;
 template <> struct VT<double> {
     using v4 = double4;
@@ -22,6 +24,7 @@ template <> struct VT<float> {
 template <typename R>
 struct DevPtrs {
     int nc, nf, nb, maxS, maxC, ntiles;
+    int boundaryFirstOrder;    // extension: first-order faces in cells with a boundary face
     typename VT<R>::v4* W;     // nc+nb primitive p, Ux, Uy, T
     double4* U;                // nc conserved rho, rhoUx, rhoUy, rhoE
     double4* Uold;             // nc
@@ -43,14 +46,15 @@ struct DevPtrs {
     const int2* fcell;         // nf own, nei
     const double4* fgeo;       // 2*nf: (Sfx, Sfy, magSf, deltaR), (dxo, dyo, dxn, dyn)
     typename VT<R>::v4* fv;    // 2*nf reconstructed face values, [2f + side]
+    R* fvT;                    // 2*nf reconstructed face temperatures (the part the BVD test reads)
     double4* flux;             // nf + nb
     const int32_t* bcell;      // nb
     const uint8_t* blabel;     // nb
     const double4* bgeo;       // nb (Sfx, Sfy, magSf, 0)
     const double4* bval;       // nb fixed values of label 6
     double* dt;                // device time step
-    unsigned long long* dt_accum;
-    unsigned int* dt_count;
+    double* dt_part;           // kCflBlocks partial minima of the CFL pass
+    unsigned int* dt_count;    // blocks of the CFL pass done
 };
 
 template <typename R> void launch_lsq(const DevPtrs<R>& p, hipStream_t s);

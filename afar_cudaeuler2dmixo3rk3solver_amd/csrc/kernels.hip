@@ -71,44 +71,62 @@ __global__ void k_lsq(DevPtrs<R> p) {
 
 // ---------------------------------------------------------------- K4 time step
 
+__device__ __forceinline__ double block_min_256(double v, double* wmin) {
+    // wavefront min through cross-lane swizzles, then across the 4 waves
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double m = wmin[0];
+    for (int w = 1; w < 4; ++w) m = wmin[w] < m ? wmin[w] : m;
+    __syncthreads();
+    return m;
+}
+
+// Each block reduces a contiguous chunk of faces (stride loop) to one
+// partial minimum; the last block to finish reduces the partials and stores
+// the time step on the device.  At most kCflBlocks partials, so only that
+// many same-address atomics per step.
 template <typename R>
-__global__ __launch_bounds__(256) void k_cfl(DevPtrs<R> p, double R_, double Cv, double CFL) {
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int f = tile * 256 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_cfl(DevPtrs<R> p, double R_, double Cv, double CFL, int chunk) {
+    const int blk = xcd_tile(blockIdx.x, gridDim.x);
+    const int f0 = blk * chunk;
+    const int f1 = min(p.nf, f0 + chunk);
     double dtf = 1e5;
-    if (f < p.nf) {
+    for (int f = f0 + threadIdx.x; f < f1; f += 256) {
         const int2 oc = p.fcell[f];
         const double4 g0 = p.fgeo[2 * f], g1 = p.fgeo[2 * f + 1];
         const double Sfx = g0.x, Sfy = g0.y, magSf = g0.z, deltaR = g0.w;
         const double fraction = afar_hypot(g1.x, g1.y) / deltaR;
         const typename VT<R>::v4 wo = p.W[oc.x], wn = p.W[oc.y];
         const double uxo = wo.y, uyo = wo.z, To = wo.w, uxn = wn.y, uyn = wn.z, Tn = wn.w;
-        dtf = CFL * deltaR * magSf /
-              ((1.0 - fraction) * fabs(uxo * Sfx + uyo * Sfy) + fraction * fabs(uxn * Sfx + uyn * Sfy) +
-               sqrt((R_ / Cv + 1.0) * R_ * ((1.0 - fraction) * To + fraction * Tn)) * magSf);
-    }
-    // wavefront min through cross-lane swizzles, then across the 4 waves
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(dtf, off);
-        dtf = dtf < o ? dtf : o;
+        const double d = CFL * deltaR * magSf /
+                         ((1.0 - fraction) * fabs(uxo * Sfx + uyo * Sfy) + fraction * fabs(uxn * Sfx + uyn * Sfy) +
+                          sqrt((R_ / Cv + 1.0) * R_ * ((1.0 - fraction) * To + fraction * Tn)) * magSf);
+        dtf = d < dtf ? d : dtf;  // a NaN face value never replaces the minimum
     }
     __shared__ double wmin[4];
     __shared__ int last;
-    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = dtf;
-    __syncthreads();
+    const double m = block_min_256(dtf, wmin);
     if (threadIdx.x == 0) {
-        double m = wmin[0];
-        for (int w = 1; w < 4; ++w) m = m < wmin[w] ? m : wmin[w];
-        atomicMin(p.dt_accum, (unsigned long long)afar_bits(m));
+        __hip_atomic_store(&p.dt_part[blockIdx.x], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         const unsigned t = atomicAdd(p.dt_count, 1u);
         last = (t == gridDim.x - 1);
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        const unsigned long long v = atomicExch(p.dt_accum, (unsigned long long)afar_bits(1e5));
-        *p.dt = afar_from_bits(v);
+    if (!last) return;
+    __threadfence();
+    double v = 1e5;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += 256) {
+        const double o = __hip_atomic_load(&p.dt_part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = o < v ? o : v;
+    }
+    v = block_min_256(v, wmin);
+    if (threadIdx.x == 0) {
+        *p.dt = v;
         atomicExch(p.dt_count, 0u);
     }
 }
@@ -196,6 +214,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
             o.z = wc.z + guxy * dx + guyy * dy;
             o.w = wc.w + gTx * dx + gTy * dy;
             p.fv[2 * f + isNei] = o;
+            p.fvT[2 * f + isNei] = o.w;
         }
         p.smooth[c] = 0;
         return;
@@ -243,6 +262,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
         o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
         isNotTrouble = phif > (R)0;
         p.fv[2 * f + isNei] = o;
+            p.fvT[2 * f + isNei] = o.w;
     }
     const R nt = (R)isNotTrouble;
     p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
@@ -302,7 +322,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
         }
         const double2 d = p.slot_d[sell(c, fi, maxC)];
         const R dx = (R)d.x, dy = (R)d.y;
-        const R TfOwn = p.fv[2 * f].w, TfNei = p.fv[2 * f + 1].w;
+        const R TfOwn = p.fvT[2 * f], TfNei = p.fvT[2 * f + 1];
         const R Tf_cur = (R)isNei * TfNei + (R)(1 - isNei) * TfOwn;
         const R Tf_nei = (R)(1 - isNei) * TfNei + (R)isNei * TfOwn;
         const R lim = to_bj(limc.w);
@@ -312,7 +332,16 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
         const int isLimited = (lim < K<R>::VALID_LIM_FACTOR) + (shockTypeExt > 0);
         bvd = max(bvd, isNotFlat * isLimited * typeval);
     }
-    const int finalType = (shockType > 0) * (shockTypeExt > 1) * (-2) + (bvd > 0);
+    int finalType = (shockType > 0) * (shockTypeExt > 1) * (-2) + (bvd > 0);
+    if (p.boundaryFirstOrder && bfaces > 0) {
+        // extension: a cell with a boundary face takes the first-order state
+        // on its faces (TVD with zero limiter), generalising the reference's
+        // corner-cell rule; shock types keep their flux choice
+        finalType = finalType ? finalType : 1;
+        p.ftype[c] = (int8_t)finalType;
+        p.lim[c] = v4{(R)0, (R)0, (R)0, (R)0};
+        return;
+    }
     p.ftype[c] = (int8_t)finalType;
     const int isShock = finalType < 0;
     p.lim[c] = v4{transform_limiter(limc.x, isShock), transform_limiter(limc.y, isShock),
@@ -559,8 +588,9 @@ static inline int pad8(int n) { return ((n + 7) / 8) * 8; }
 
 template <typename R>
 void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s) {
-    const int nt = pad8(std::max(1, (p.nf + 255) / 256));
-    hipLaunchKernelGGL(k_cfl<R>, dim3(nt), dim3(256), 0, s, p, Rg, Cv, CFL);
+    const int nt = pad8(std::min(kCflBlocks, std::max(1, (p.nf + 255) / 256)));
+    const int chunk = (p.nf + nt - 1) / nt;
+    hipLaunchKernelGGL(k_cfl<R>, dim3(nt), dim3(256), 0, s, p, Rg, Cv, CFL, chunk);
 }
 
 template <typename R>

@@ -158,45 +158,29 @@ class SolverImpl final : public DeviceSolver {
         return ms;
     }
 
-    // Times every launch of `reps` auto-dt steps with events on the solver's stream.
+    // Average duration of each kernel: `reps` back-to-back launches of that
+    // kernel between two events on the solver's stream (the bvd and update
+    // launches keep transforming the state, so call this after timed work).
     void profile(double* ms, int* count, int reps) override {
-        std::vector<hipEvent_t> ev;
-        std::vector<int> kind;
-        auto mark = [&](int k) {
-            hipEvent_t e;
-            check_hip(hipEventCreate(&e), "hipEventCreate");
-            check_hip(hipEventRecord(e, stream_), "hipEventRecord");
-            ev.push_back(e);
-            kind.push_back(k);
-        };
         synchronize();
-        mark(-1);
-        for (int r = 0; r < reps; ++r) {
-            launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
-            mark(kKernCfl);
-            for (int s = 0; s < 3; ++s) {
-                launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_);
-                mark(kKernRecon);
-                launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
-                mark(kKernBvd);
-                launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
-                mark(kKernFlux);
-                launch_update<R>(P_, cfg_.R, gamma(), 0.0, P_.dt, b1(s), b2(s), b3(s), s == 2, stream_);
-                mark(kKernUpdate);
-            }
-        }
-        check_hip(hipEventSynchronize(ev.back()), "hipEventSynchronize");
         for (int k = 0; k < kKernKinds; ++k) {
-            ms[k] = 0.0;
-            count[k] = 0;
-        }
-        for (size_t i = 1; i < ev.size(); ++i) {
+            check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+            for (int r = 0; r < reps; ++r) {
+                switch (k) {
+                    case kKernCfl: launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_); break;
+                    case kKernRecon: launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_); break;
+                    case kKernBvd: launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_); break;
+                    case kKernFlux: launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_); break;
+                    default: launch_update<R>(P_, cfg_.R, gamma(), 0.0, P_.dt, b1(1), b2(1), b3(1), 0, stream_);
+                }
+            }
+            check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
+            check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
             float t = 0.f;
-            check_hip(hipEventElapsedTime(&t, ev[i - 1], ev[i]), "hipEventElapsedTime");
-            ms[kind[i]] += t;
-            count[kind[i]] += 1;
+            check_hip(hipEventElapsedTime(&t, ev0_, ev1_), "hipEventElapsedTime");
+            ms[k] = t / reps;
+            count[k] = k == kKernCfl ? 1 : 3;  // launches per RK3 step
         }
-        for (auto e : ev) (void)hipEventDestroy(e);
     }
 
     const DevLayout& layout() const override { return L_; }
@@ -314,6 +298,7 @@ class SolverImpl final : public DeviceSolver {
         P_.maxS = L_.maxS;
         P_.maxC = L_.maxC;
         P_.ntiles = L_.ntiles;
+        P_.boundaryFirstOrder = cfg_.boundaryFirstOrder;
         m_W.alloc(sizeof(v4) * (nc + nb));
         P_.W = m_W.as<v4>();
         m_U.alloc(sizeof(double4) * nc);
@@ -353,23 +338,23 @@ class SolverImpl final : public DeviceSolver {
         P_.fgeo = reinterpret_cast<const double4*>(put(m_fgeo, L_.fgeo.data(), 8 * nf));
         m_fv.alloc(sizeof(v4) * 2 * nf);
         P_.fv = m_fv.as<v4>();
+        m_fvT.alloc(sizeof(R) * 2 * nf);
+        P_.fvT = m_fvT.as<R>();
         m_flux.alloc(sizeof(double4) * (nf + nb));
         P_.flux = m_flux.as<double4>();
         P_.bcell = put(m_bcell, L_.bcell.data(), nb);
         P_.blabel = put(m_blabel, L_.blabel.data(), nb);
         P_.bgeo = reinterpret_cast<const double4*>(put(m_bgeo, L_.bgeo.data(), 4 * nb));
         P_.bval = reinterpret_cast<const double4*>(put(m_bval, L_.bval.data(), 4 * nb));
-        m_dt.alloc(sizeof(double) + sizeof(unsigned long long) + sizeof(unsigned int) + 16);
+        // dt, the ticket counter (own 256-byte line) and the CFL partials
+        m_dt.alloc(512 + sizeof(double) * kCflBlocks);
         P_.dt = m_dt.as<double>();
-        P_.dt_accum = reinterpret_cast<unsigned long long*>(m_dt.as<char>() + 8);
-        P_.dt_count = reinterpret_cast<unsigned int*>(m_dt.as<char>() + 16);
+        P_.dt_count = reinterpret_cast<unsigned int*>(m_dt.as<char>() + 256);
+        P_.dt_part = reinterpret_cast<double*>(m_dt.as<char>() + 512);
         {
             const double init_dt = 1e5;
-            unsigned long long acc;
-            memcpy(&acc, &init_dt, 8);
             unsigned int zero = 0;
             check_hip(hipMemcpy(P_.dt, &init_dt, 8, hipMemcpyHostToDevice), "dt init");
-            check_hip(hipMemcpy(P_.dt_accum, &acc, 8, hipMemcpyHostToDevice), "dt init");
             check_hip(hipMemcpy(P_.dt_count, &zero, 4, hipMemcpyHostToDevice), "dt init");
         }
         upload_state();
@@ -389,7 +374,7 @@ class SolverImpl final : public DeviceSolver {
     size_t lds_recon_ = 0, lds_bvd_ = 0;
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fv, m_flux, m_bcell,
-        m_blabel, m_bgeo, m_bval, m_dt;
+        m_blabel, m_bgeo, m_bval, m_dt, m_fvT;
 };
 
 }  // namespace

@@ -16,6 +16,7 @@ struct SolverConfig {
     int precision = 0;  // 0 FP64, 1 FP32-mixed
     int exactRK = 0;
     int tile = 256;
+    int boundaryFirstOrder = 0;
 };
 
 // kernel kinds reported by profile()

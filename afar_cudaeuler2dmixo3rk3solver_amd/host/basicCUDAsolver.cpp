@@ -170,6 +170,7 @@ afar_schemes readSchemes(const dictionary& fv, const dictionary& control) {
     s.precision = prec == "mixed" ? 1 : 0;
     s.exactRK = fv.lookupOrDefault("CUDA.exactRK", false);
     s.tile = (int32_t)fv.lookupOrDefault("CUDA.tile", 256L);
+    s.boundaryFirstOrder = fv.lookupOrDefault("CUDA.boundaryFirstOrder", false);
     return s;
 }
 

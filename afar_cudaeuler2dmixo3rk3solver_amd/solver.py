@@ -18,8 +18,10 @@ STENCIL_TYPES = {"faceNeighbour": 0, "edgeNeighbour": 1}
 
 
 def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilType="edgeNeighbour",
-            recalcBasis=True, cOrder=5, fOrder=4, precision="fp64", exactRK=False, tile=256):
-    """afar_schemes with the reference's key names and defaults."""
+            recalcBasis=True, cOrder=5, fOrder=4, precision="fp64", exactRK=False, tile=256,
+            boundaryFirstOrder=False):
+    """afar_schemes with the reference's key names and defaults (the last
+    four are this library's extensions)."""
     s = Schemes()
     lib().afar_schemes_default(C.byref(s))
     s.R, s.Cv, s.maxCo, s.shapeFactor = R, Cv, maxCo, shapeFactor
@@ -29,6 +31,7 @@ def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilTy
     s.precision = {"fp64": 0, "mixed": 1, 0: 0, 1: 1}[precision]
     s.exactRK = int(bool(exactRK))
     s.tile = tile
+    s.boundaryFirstOrder = int(bool(boundaryFirstOrder))
     return s
 
 

@@ -58,6 +58,9 @@ typedef struct afar_schemes {
     int32_t precision;   /* 0 all-FP64, 1 FP32 reconstruction/limiter/flux with FP64 state */
     int32_t exactRK;     /* 0 the reference's RK3 literals (GPUevolve.cu:223-225), 1 exact 1/3, 2/3 */
     int32_t tile;        /* cells per LDS tile / workgroup: 64, 128 or 256 (0 = 256) */
+    int32_t boundaryFirstOrder; /* extension, 0 = reference behaviour: cells with a boundary face
+                                   use first-order face states (the reference does this only for
+                                   cells with two boundary faces, cudaLimiter.h:419-429) */
 } afar_schemes;
 
 const char* afar_last_error(void);

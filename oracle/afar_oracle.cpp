@@ -59,6 +59,7 @@ typedef struct oracle_fields {
     double* faceFD;                  // 8*nf: [2f+side] * 4
     double* Flux;                    // 4*(nf+nb)
     double* Res;                     // 4*nc
+    int32_t boundaryFirstOrder;      // library extension (0 = reference behaviour)
 } oracle_fields;
 
 }  // extern "C"
@@ -424,15 +425,17 @@ void oracle_bvd(oracle_fields* f) {
             const int isLimited = (l < VALID_LIM_FACTOR) + (shockType_extend > 0);
             BVDtype = std::max(BVDtype, isNotFlat * isLimited * typeval);
         }
-        const int finalType = (shockType > 0) * (shockType_extend > 1) * (-2) + (BVDtype > 0);
+        int finalType = (shockType > 0) * (shockType_extend > 1) * (-2) + (BVDtype > 0);
+        if (f->boundaryFirstOrder && bfacesNum > 0) {
+            // extension: first-order face states in cells with a boundary face
+            finalType = finalType ? finalType : 1;
+            lim[0] = lim[1] = lim[2] = lim[3] = 0.0;
+            f->shockIndicator[2 * (size_t)celli + 1] = (int8_t)finalType;
+            continue;
+        }
         f->shockIndicator[2 * (size_t)celli + 1] = (int8_t)finalType;
-    }
-    // limiter transform after every cell's type is known (the reference does it
-    // per thread after its own reads; no other cell reads this cell's limiter in K6)
-#pragma omp parallel for schedule(static)
-    for (int celli = 0; celli < f->nc; ++celli) {
-        const bool isShock = f->shockIndicator[2 * (size_t)celli + 1] < 0;
-        double* lim = &f->limiter[4 * (size_t)celli];
+        // limiter transform (only this cell reads its own limiter in K6)
+        const bool isShock = finalType < 0;
         for (int k = 0; k < 4; ++k) lim[k] = transform_limiter(lim[k], isShock);
     }
 }

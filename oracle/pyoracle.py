@@ -26,6 +26,7 @@ class OracleFields(C.Structure):
         ("RBFbasis", P(C.c_double)), ("FSHIFT", P(C.c_double)), ("BVALUE", P(C.c_double)),
         ("matrix", P(C.c_double)), ("gradFD", P(C.c_double)), ("limiter", P(C.c_double)),
         ("shockIndicator", P(C.c_int8)), ("faceFD", P(C.c_double)), ("Flux", P(C.c_double)), ("Res", P(C.c_double)),
+        ("boundaryFirstOrder", C.c_int32),
     ]
 
 
@@ -76,7 +77,7 @@ class OracleState:
 
     RK_LITERAL = ((1.0, 0.75, 0.333333), (0.0, 0.25, 0.666667), (1.0, 0.25, 0.666667))
 
-    def __init__(self, ref, R, Cv):
+    def __init__(self, ref, R, Cv, boundary_first_order=False):
         s = ref["sizes"]
         nc, nf, nb, maxS, maxC = s["nc"], s["nf"], s["nb"], s["maxS"], s["maxC"]
         self.R, self.Cv = R, Cv
@@ -107,6 +108,7 @@ class OracleState:
         f.FSHIFT = _p(a["faceShift"])
         f.BVALUE = _p(a["boundaryValue"])
         f.shockIndicator = _p(a["shockIndicator"], C.c_int8)
+        f.boundaryFirstOrder = int(bool(boundary_first_order))
         self.f = f
         lib().oracle_least_square_matrix(C.byref(f))
 
