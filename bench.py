@@ -93,11 +93,23 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--faithful-boundary", action="store_true",
                     help="riemann case without the boundaryFirstOrder extension (loses positivity)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 8, "bytes_written": n * 8, "copies": 3}))
+        return
 
     import afar_cudaeuler2dmixo3rk3solver_amd as A
     from afar_cudaeuler2dmixo3rk3solver_amd import cases
