@@ -32,6 +32,7 @@ class Schemes(C.Structure):
         ("exactRK", C.c_int32),
         ("tile", C.c_int32),
         ("boundaryFirstOrder", C.c_int32),
+        ("reconLanes", C.c_int32),
     ]
 
 
@@ -84,7 +85,7 @@ SIGNATURES = {
     "afar_mesh_generate_corner": ([C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                    C.c_double, C.c_double, _D, C.c_int32, P(_VP)], C.c_int),
     "afar_mesh_generate_riemann": ([C.c_int32, C.c_double, C.c_uint64, C.c_double, P(_VP)], C.c_int),
-    "afar_mesh_generate_periodic": ([C.c_int32, C.c_double, C.c_double, C.c_uint64, P(_VP)], C.c_int),
+    "afar_mesh_generate_periodic": ([C.c_int32, C.c_double, C.c_double, C.c_uint64, C.c_double, P(_VP)], C.c_int),
     "afar_mesh_destroy": ([_VP], None),
     "afar_mesh_set_patch": ([_VP, C.c_int32, C.c_int32, _D], C.c_int),
     "afar_mesh_sizes": ([_VP, _I64], C.c_int),
@@ -105,6 +106,11 @@ SIGNATURES = {
     "afar_solver_evolve_auto": ([_VP, C.c_int32], C.c_int),
     "afar_solver_last_dt": ([_VP, _D], C.c_int),
     "afar_solver_synchronize": ([_VP], C.c_int),
+    "afar_solver_set_time": ([_VP, C.c_double, C.c_double], C.c_int),
+    "afar_solver_get_time": ([_VP, _D], C.c_int),
+    "afar_solver_residual_norm": ([_VP, _D], C.c_int),
+    "afar_batch_set_time": ([_VP, C.c_int32, C.c_double, C.c_double], C.c_int),
+    "afar_batch_get_time": ([_VP, _D], C.c_int),
     "afar_solver_write_field": ([_VP, _D, _D, _I8], C.c_int),
     "afar_solver_get_conserved": ([_VP, _D], C.c_int),
     "afar_solver_set_conserved": ([_VP, _D], C.c_int),
@@ -114,6 +120,18 @@ SIGNATURES = {
     "afar_solver_run_stage": ([_VP, C.c_int32, C.c_double, C.c_int32], C.c_int),
     "afar_solver_fetch": ([_VP, C.c_char_p, _VP, C.c_size_t], C.c_int),
     "afar_solver_destroy": ([_VP], None),
+    "afar_batch_create": ([C.c_int32, P(_VP), P(Schemes), P(_D), P(_VP)], C.c_int),
+    "afar_batch_evolve": ([_VP, C.c_int32], C.c_int),
+    "afar_batch_time_steps": ([_VP, C.c_int32, _D], C.c_int),
+    "afar_batch_synchronize": ([_VP], C.c_int),
+    "afar_batch_reset": ([_VP], C.c_int),
+    "afar_batch_sizes": ([_VP, _I64], C.c_int),
+    "afar_batch_cell_offsets": ([_VP, _I32], C.c_int),
+    "afar_batch_last_dt": ([_VP, _D], C.c_int),
+    "afar_batch_get_conserved": ([_VP, C.c_int32, _D], C.c_int),
+    "afar_batch_write_field": ([_VP, C.c_int32, _D, _I8], C.c_int),
+    "afar_batch_profile": ([_VP, C.c_int32, _D, _I32], C.c_int),
+    "afar_batch_destroy": ([_VP], None),
     "afar_set_device_field_data": ([P(_VP), P(FieldPointer), C.c_int32, C.c_int32, C.c_int32], C.c_int),
     "afar_device_configure": ([_VP, C.c_int32, C.c_int32], C.c_int),
     "afar_set_device_stencil_data": ([_VP, P(FieldPointer), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,

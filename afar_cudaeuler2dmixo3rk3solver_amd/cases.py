@@ -78,7 +78,72 @@ def vortex_exact(xy, t, beta=5.0, gamma=1.4, L=10.0, x0=5.0, y0=5.0, u_inf=1.0, 
     return W, rho
 
 
-def vortex_case(n=64, seed=0, jitter=0.15, L=10.0, R=R_REF):
-    mesh = Mesh.periodic(n=n, L=L, jitter=jitter, seed=seed)
-    W, _ = vortex_exact(_centroids(mesh), 0.0, L=L, R=R)
-    return mesh, W
+def triangle_rule(n):
+    """Conical product rule on the triangle (0,0), (1,0), (0,1): Gauss-Legendre
+    in u and v mapped by (xi, eta) = (u, v (1 - u)) with Jacobian (1 - u);
+    n*n points, exact for polynomials of degree 2n - 2 (weights sum to 1/2)."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    u, wu = 0.5 * (x + 1.0), 0.5 * w
+    U, V = np.meshgrid(u, u, indexing="ij")
+    WU, WV = np.meshgrid(wu, wu, indexing="ij")
+    return U.ravel(), (V * (1.0 - U)).ravel(), (WU * WV * (1.0 - U)).ravel()
+
+
+def cell_average(mesh, fn, n=5):
+    """Cell averages of fn(x, y) -> (npts, m) over each polygonal cell (fan
+    triangulation, conical product rule per triangle), in the caller's order."""
+    vptr, vxy = mesh.polygons()
+    nc = mesh.nc
+    nv = np.diff(vptr)
+    xi, eta, wq = triangle_rule(n)
+    acc = None
+    area = np.zeros(nc)
+    v0 = vxy[vptr[:-1]]
+    for k in range(1, int(nv.max()) - 1):
+        live = nv > k + 1
+        idx = np.where(live, vptr[:-1] + k, vptr[:-1])
+        a = vxy[idx] - v0
+        b = vxy[np.where(live, vptr[:-1] + k + 1, vptr[:-1])] - v0
+        det = np.where(live, a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0], 0.0)
+        pts = v0[:, None, :] + xi[None, :, None] * a[:, None, :] + eta[None, :, None] * b[:, None, :]
+        vals = fn(pts[..., 0].ravel(), pts[..., 1].ravel())
+        vals = vals.reshape(nc, len(wq), -1)
+        part = np.einsum("q,cqm->cm", wq, vals) * det[:, None]
+        acc = part if acc is None else acc + part
+        area += 0.5 * det
+    avg = acc / area[:, None]
+    out = np.empty_like(avg)
+    out[mesh.permutation()] = avg
+    return out
+
+
+def vortex_conserved(x, y, t=0.0, beta=5.0, gamma=1.4, L=10.0, x0=5.0, y0=5.0, u_inf=1.0, v_inf=1.0):
+    """Conserved (rho, rho u, rho v, rho E) of the isentropic vortex at points
+    (x, y); E = p/(rho (gamma - 1)) + |u|^2/2."""
+    W, rho = vortex_exact(np.stack([x, y], axis=1), t, beta, gamma, L, x0, y0, u_inf, v_inf, R=1.0)
+    p, u, v = W[:, 0], W[:, 1], W[:, 2]
+    return np.stack([rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)], axis=1)
+
+
+def primitive_from_conserved(U, R=R_REF, Cv=CV_REF):
+    """(p, Ux, Uy, T) whose conserved state, as the solver forms it, is U."""
+    gamma = R / Cv + 1.0
+    rho = U[:, 0]
+    ux, uy = U[:, 1] / rho, U[:, 2] / rho
+    p = (gamma - 1.0) * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / rho)
+    return np.stack([p, ux, uy, p / (R * rho)], axis=1)
+
+
+def vortex_case(n=64, seed=0, jitter=0.15, L=10.0, R=R_REF, averaged=False, quad=5, split_p=1.0):
+    """Isentropic vortex (strength 5) on the doubly periodic [0, L]^2 mesh of
+    2 n^2 jittered triangles.  averaged: the initial state is the cell average
+    of the conserved variables (quad x quad conical rule per triangle); else
+    point values at the centroids.  Returns mesh, W0 (and with averaged, the
+    exact cell-average conserved state, which is also the exact solution at
+    t = L, one period of the (1, 1) advection)."""
+    mesh = Mesh.periodic(n=n, L=L, jitter=jitter, seed=seed, split_p=split_p)
+    if not averaged:
+        W, _ = vortex_exact(_centroids(mesh), 0.0, L=L, R=R)
+        return mesh, W
+    Ubar = cell_average(mesh, lambda x, y: vortex_conserved(x, y, L=L), n=quad)
+    return mesh, primitive_from_conserved(Ubar, R), Ubar

@@ -119,7 +119,7 @@ Mesh build_mesh(const PolyMesh& pm, bool renumber);
 PolyMesh generate_corner(int nx, int ny, double lx, double ly, double jitter, uint64_t seed,
                          double corner_x, double angle_deg, const double inflow[4], int top_label);
 PolyMesh generate_riemann(int n, double jitter, uint64_t seed, double split_p);
-PolyMesh generate_periodic(int n, double L, double jitter, uint64_t seed);
+PolyMesh generate_periodic(int n, double L, double jitter, uint64_t seed, double split_p);
 std::vector<int32_t> rcm_order(int n, const std::vector<int32_t>& adj_ptr, const std::vector<int32_t>& adj);
 
 // ---- precompute.cpp

@@ -79,6 +79,12 @@ struct afar_solver {
     int nc = 0, nb = 0;
 };
 
+struct afar_batch {
+    std::unique_ptr<DeviceSolver> dev;
+    BatchIndex idx;
+    std::vector<std::vector<int32_t>> perm;  // per instance: mesh cell -> caller cell
+};
+
 struct afar_device {
     RefArrays ra;
     SolverConfig cfg;
@@ -166,8 +172,8 @@ int afar_mesh_generate_riemann(int32_t n, double jitter, uint64_t seed, double s
     return guard([&] { finish_mesh(generate_riemann(n, jitter, seed, split_p), true, out); });
 }
 
-int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t seed, afar_mesh** out) {
-    return guard([&] { finish_mesh(generate_periodic(n, L, jitter, seed), true, out); });
+int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t seed, double split_p, afar_mesh** out) {
+    return guard([&] { finish_mesh(generate_periodic(n, L, jitter, seed, split_p), true, out); });
 }
 
 void afar_mesh_destroy(afar_mesh* m) { delete m; }
@@ -437,12 +443,179 @@ int afar_solver_create(const afar_mesh* h, const afar_schemes* s, const double* 
         cfg.exactRK = s->exactRK;
         cfg.tile = tile_of(s);
         cfg.boundaryFirstOrder = s->boundaryFirstOrder != 0;
+        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 4, "reconLanes must be 0, 1 or 4");
+        cfg.reconLanes = s->reconLanes;
         auto sol = std::make_unique<afar_solver>();
         sol->dev = make_device_solver(ra, cfg);
         sol->perm = h->m.perm;
         sol->nc = h->m.nc;
         sol->nb = h->m.nb;
         *out = sol.release();
+    });
+}
+
+// ---------------------------------------------------------------- batches
+
+int afar_batch_create(int32_t n, const afar_mesh* const* meshes, const afar_schemes* s, const double* const* W0,
+                      afar_batch** out) {
+    return guard([&] {
+        need(out && meshes && W0 && s && n > 0, "bad arguments");
+        auto b = std::make_unique<afar_batch>();
+        RefArrays u;
+        {
+            std::vector<RefArrays> parts(n);
+            std::vector<const RefArrays*> pp(n);
+            for (int k = 0; k < n; ++k) {
+                need(meshes[k] != nullptr && W0[k] != nullptr, "NULL mesh or state in the batch");
+                parts[k] = ref_arrays_of(meshes[k], s, W0[k]);
+                pp[k] = &parts[k];
+                b->perm.push_back(meshes[k]->m.perm);
+            }
+            u = union_ref_arrays(pp, b->idx);
+        }
+        SolverConfig cfg;
+        cfg.R = s->R;
+        cfg.Cv = s->Cv;
+        cfg.CFL = s->maxCo;
+        cfg.precision = s->precision;
+        cfg.exactRK = s->exactRK;
+        cfg.tile = tile_of(s);
+        cfg.boundaryFirstOrder = s->boundaryFirstOrder != 0;
+        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 4, "reconLanes must be 0, 1 or 4");
+        cfg.reconLanes = s->reconLanes;
+        b->dev = make_device_solver(u, cfg, &b->idx);
+        *out = b.release();
+    });
+}
+
+int afar_batch_evolve(afar_batch* b, int32_t nsteps) {
+    return guard([&] {
+        need(b && nsteps >= 0, "bad arguments");
+        b->dev->evolve_auto(nsteps);
+    });
+}
+
+int afar_batch_time_steps(afar_batch* b, int32_t nsteps, double* ms) {
+    return guard([&] {
+        need(b && ms && nsteps >= 0, "bad arguments");
+        *ms = b->dev->time_steps(nsteps, true);
+    });
+}
+
+int afar_batch_synchronize(afar_batch* b) {
+    return guard([&] {
+        need(b != nullptr, "NULL batch");
+        b->dev->synchronize();
+    });
+}
+
+int afar_batch_reset(afar_batch* b) {
+    return guard([&] {
+        need(b != nullptr, "NULL batch");
+        b->dev->reset_state();
+    });
+}
+
+int afar_batch_sizes(afar_batch* b, int64_t* out) {
+    return guard([&] {
+        need(b && out, "NULL argument");
+        const DevLayout& L = b->dev->layout();
+        out[0] = b->idx.ninst();
+        out[1] = L.nc;
+        out[2] = L.nf;
+        out[3] = L.nb;
+        out[4] = L.tile;
+        out[5] = L.maxS;
+        out[6] = L.maxC;
+        out[7] = L.maxHalo;
+    });
+}
+
+int afar_batch_cell_offsets(afar_batch* b, int32_t* off) {
+    return guard([&] {
+        need(b && off, "NULL argument");
+        memcpy(off, b->idx.cell_off.data(), sizeof(int32_t) * b->idx.cell_off.size());
+    });
+}
+
+int afar_batch_last_dt(afar_batch* b, double* dt) {
+    return guard([&] {
+        need(b && dt, "NULL argument");
+        b->dev->last_dts(dt);
+    });
+}
+
+int afar_batch_get_conserved(afar_batch* b, int32_t inst, double* U) {
+    return guard([&] {
+        need(b && U && inst >= 0 && inst < b->idx.ninst(), "bad arguments");
+        const int c0 = b->idx.cell_off[inst], n = b->idx.cell_off[inst + 1] - c0;
+        std::vector<double> u(4 * (size_t)n);
+        b->dev->get_conserved_range(c0, n, u.data());
+        const auto& perm = b->perm[inst];
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 4; ++k) U[4 * (size_t)perm[i] + k] = u[4 * (size_t)i + k];
+    });
+}
+
+int afar_batch_write_field(afar_batch* b, int32_t inst, double* W, int8_t* shock) {
+    return guard([&] {
+        need(b && inst >= 0 && inst < b->idx.ninst(), "bad arguments");
+        const int c0 = b->idx.cell_off[inst], n = b->idx.cell_off[inst + 1] - c0;
+        std::vector<double> w(4 * (size_t)n);
+        std::vector<int8_t> t(n);
+        b->dev->get_primitive_range(c0, n, w.data(), t.data());
+        const auto& perm = b->perm[inst];
+        for (int i = 0; i < n; ++i) {
+            if (W)
+                for (int k = 0; k < 4; ++k) W[4 * (size_t)perm[i] + k] = w[4 * (size_t)i + k];
+            if (shock) shock[perm[i]] = t[i];
+        }
+    });
+}
+
+int afar_batch_profile(afar_batch* b, int32_t reps, double* ms, int32_t* count) {
+    return guard([&] {
+        need(b && ms && count && reps > 0, "bad arguments");
+        int c[kKernKinds];
+        b->dev->profile(ms, c, reps);
+        for (int k = 0; k < kKernKinds; ++k) count[k] = c[k];
+    });
+}
+
+void afar_batch_destroy(afar_batch* b) { delete b; }
+
+int afar_solver_set_time(afar_solver* h, double t, double t_end) {
+    return guard([&] {
+        need(h != nullptr, "NULL solver");
+        h->dev->set_time(-1, t, t_end);
+    });
+}
+
+int afar_solver_get_time(afar_solver* h, double* t) {
+    return guard([&] {
+        need(h && t, "NULL argument");
+        h->dev->get_time(t);
+    });
+}
+
+int afar_solver_residual_norm(afar_solver* h, double* norm) {
+    return guard([&] {
+        need(h && norm, "NULL argument");
+        *norm = h->dev->residual_norm();
+    });
+}
+
+int afar_batch_set_time(afar_batch* b, int32_t inst, double t, double t_end) {
+    return guard([&] {
+        need(b != nullptr, "NULL batch");
+        b->dev->set_time(inst, t, t_end);
+    });
+}
+
+int afar_batch_get_time(afar_batch* b, double* t) {
+    return guard([&] {
+        need(b && t, "NULL argument");
+        b->dev->get_time(t);
     });
 }
 

@@ -24,6 +24,7 @@ template <> struct VT<float> {
 template <typename R>
 struct DevPtrs {
     int nc, nf, nb, maxS, maxC, ntiles;
+    int lds_sten_off;          // byte offset of the stencil ids in the reconstruction kernels' LDS
     int boundaryFirstOrder;    // extension: first-order faces in cells with a boundary face
     typename VT<R>::v4* W;     // nc+nb primitive p, Ux, Uy, T
     double4* U;                // nc conserved rho, rhoUx, rhoUy, rhoE
@@ -52,16 +53,41 @@ struct DevPtrs {
     const uint8_t* blabel;     // nb
     const double4* bgeo;       // nb (Sfx, Sfy, magSf, 0)
     const double4* bval;       // nb fixed values of label 6
-    double* dt;                // device time step
-    double* dt_part;           // kCflBlocks partial minima of the CFL pass
+    double* dt;                // device time step of each instance (ninst)
+    double* dt_part;           // dt_cap partial minima of the CFL pass
     unsigned int* dt_count;    // blocks of the CFL pass done
+    int dt_cap;                // capacity of dt_part: kCflBlocks or two per flux block, whichever is larger
+    // Batches: ninst independent meshes stored one after the other (cells,
+    // internal faces, boundary faces each contiguous per instance).  Each has
+    // its own CFL time step.  A single mesh is ninst = 1 with cinst = nullptr.
+    int ninst;
+    const int32_t* cinst;      // nc instance of each cell, or nullptr
+    const int32_t* finst_off;  // ninst+1 first internal face of each instance
+    // Simulated time of each instance on the device path: tctl[2i] = t,
+    // tctl[2i+1] = end time.  The device time step is clipped to land on the
+    // end time; once there, steps leave the state unchanged (time step 0).
+    double* tctl;
+    double* res_part;          // partial sums of the residual norm (one per update block)
 };
 
 template <typename R> void launch_lsq(const DevPtrs<R>& p, hipStream_t s);
 template <typename R> void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s);
-template <typename R> void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s);
+// lanes: 1 = one thread per cell (any mesh); 4 = four threads per cell
+// (meshes whose cells have at most four faces).
+template <typename R> void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s, int lanes);
 template <typename R> void launch_bvd(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s);
 template <typename R> void launch_flux(const DevPtrs<R>& p, double Rg, double Cv, hipStream_t s);
+// Flux pass that also reduces the CFL bound of the current state (the state
+// the step starts from) to per-block minima in p.dt_part; launch_dt_reduce
+// then stores their minimum in *p.dt.
+template <typename R> void launch_flux_dt(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s);
+// Grid size of the flux kernels (blocks of 256 faces, a multiple of 8).
+int flux_blocks(int nf, int nb);
+template <typename R> void launch_dt_reduce(const DevPtrs<R>& p, hipStream_t s);
+// Sum over cells of (density residual / volume)^2 from the current fluxes
+// into *out (device); res_part needs residual_blocks(nc) entries.
+int residual_blocks(int nc);
+template <typename R> void launch_residual_norm(const DevPtrs<R>& p, double* out, hipStream_t s);
 template <typename R>
 void launch_update(const DevPtrs<R>& p, double Rg, double gamma, double dt, const double* dt_ptr, double b1, double b2,
                    double b3, int last, hipStream_t s);

@@ -85,6 +85,18 @@ __device__ __forceinline__ double block_min_256(double v, double* wmin) {
     return m;
 }
 
+// Time-step bound of one internal face (cudaFieldOperation.h:124-180).
+template <typename V4>
+__device__ __forceinline__ double cfl_face_dt(const double4& g0, const double4& g1, const V4& wo, const V4& wn,
+                                              double R_, double Cv, double CFL) {
+    const double Sfx = g0.x, Sfy = g0.y, magSf = g0.z, deltaR = g0.w;
+    const double fraction = afar_hypot(g1.x, g1.y) / deltaR;
+    const double uxo = wo.y, uyo = wo.z, To = wo.w, uxn = wn.y, uyn = wn.z, Tn = wn.w;
+    return CFL * deltaR * magSf /
+           ((1.0 - fraction) * fabs(uxo * Sfx + uyo * Sfy) + fraction * fabs(uxn * Sfx + uyn * Sfy) +
+            sqrt((R_ / Cv + 1.0) * R_ * ((1.0 - fraction) * To + fraction * Tn)) * magSf);
+}
+
 // Each block reduces a contiguous chunk of faces (stride loop) to one
 // partial minimum; the last block to finish reduces the partials and stores
 // the time step on the device.  At most kCflBlocks partials, so only that
@@ -97,14 +109,7 @@ __global__ __launch_bounds__(256) void k_cfl(DevPtrs<R> p, double R_, double Cv,
     double dtf = 1e5;
     for (int f = f0 + threadIdx.x; f < f1; f += 256) {
         const int2 oc = p.fcell[f];
-        const double4 g0 = p.fgeo[2 * f], g1 = p.fgeo[2 * f + 1];
-        const double Sfx = g0.x, Sfy = g0.y, magSf = g0.z, deltaR = g0.w;
-        const double fraction = afar_hypot(g1.x, g1.y) / deltaR;
-        const typename VT<R>::v4 wo = p.W[oc.x], wn = p.W[oc.y];
-        const double uxo = wo.y, uyo = wo.z, To = wo.w, uxn = wn.y, uyn = wn.z, Tn = wn.w;
-        const double d = CFL * deltaR * magSf /
-                         ((1.0 - fraction) * fabs(uxo * Sfx + uyo * Sfy) + fraction * fabs(uxn * Sfx + uyn * Sfy) +
-                          sqrt((R_ / Cv + 1.0) * R_ * ((1.0 - fraction) * To + fraction * Tn)) * magSf);
+        const double d = cfl_face_dt(p.fgeo[2 * f], p.fgeo[2 * f + 1], p.W[oc.x], p.W[oc.y], R_, Cv, CFL);
         dtf = d < dtf ? d : dtf;  // a NaN face value never replaces the minimum
     }
     __shared__ double wmin[4];
@@ -133,6 +138,20 @@ __global__ __launch_bounds__(256) void k_cfl(DevPtrs<R> p, double R_, double Cv,
 
 // ---------------------------------------------------------------- K5 reconstruct
 
+// Copies the stencil ids (sliced ELL, width 64) of the tile's cells into LDS:
+// the tile's waves are consecutive slices, so the block is one contiguous
+// range of (waves of the tile) * maxS * 64 ids, copied 16 bytes at a time.
+__device__ __forceinline__ void stage_stencil(uint16_t* sSt, const uint16_t* sten, int tile, int tileCells, int nc,
+                                              int maxS, int t, int nthreads) {
+    const int wpt = tileCells >> 6;  // waves per tile
+    const int w0 = tile * wpt;
+    const int nw = min(wpt, ((nc + 63) >> 6) - w0);
+    const int n16 = nw * maxS * 8;  // 16-byte words (64 ids of 2 bytes = 8 words per slice row)
+    const uint4* src = reinterpret_cast<const uint4*>(sten + (size_t)w0 * maxS * 64);
+    uint4* dst = reinterpret_cast<uint4*>(sSt);
+    for (int i = t; i < n16; i += nthreads) dst[i] = src[i];
+}
+
 template <typename R>
 __device__ __forceinline__ void limiter_process(R& lim, R& phif, R phic, R phiMax, R phiMin, R gx, R gy, R dx, R dy) {
     // cudaLimiter.h:35-50
@@ -155,8 +174,10 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
     if (tile >= p.ntiles) return;
     const int tid = threadIdx.x, c = tile * TILE + tid;
     const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
+    uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
     if (c < p.nc) sW[tid] = p.W[c];
     for (int i = tid; i < hn; i += TILE) sW[TILE + i] = p.W[p.halo[hb + i]];
+    stage_stencil(sSt, p.sten, tile, TILE, p.nc, p.maxS, tid, TILE);
     __syncthreads();
     if (c >= p.nc) return;
     const int S = p.scnt[c], C = p.ccnt[c];
@@ -166,7 +187,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
     R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
     for (int i = 0; i < C; ++i) {
         const v2 M = p.lsq[sell(c, i, maxC)];
-        const v4 wj = sW[p.sten[sell(c, i, maxS)]];
+        const v4 wj = sW[sSt[sell(tid, i, maxS)]];
         R dphi = wj.x - wc.x;
         gpx += M.x * dphi;
         gpy += M.y * dphi;
@@ -187,7 +208,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
 
     R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z, TMax = wc.w, TMin = wc.w;
     for (int i = 0; i < C; ++i) {
-        const v4 wj = sW[p.sten[sell(c, i, maxS)]];
+        const v4 wj = sW[sSt[sell(tid, i, maxS)]];
         pMax = fmax(wj.x, pMax);
         pMin = fmin(wj.x, pMin);
         uxMax = fmax(wj.y, uxMax);
@@ -234,13 +255,13 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
         const double2 e = p.slot_e[sell(c, fi, maxC)];
         const R deltaR = (R)afar_hypot(e.x, e.y);
         const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
-        const int nbl = p.sten[sell(c, fi, maxS)];
+        const int nbl = sSt[sell(tid, fi, maxS)];
 
         const R* a = p.rbf + sell(c, fi * (maxS + 1), KR);  // row stride 64
         R coef = a[0];
         R pf = coef * wc.x, uxf = coef * wc.y, uyf = coef * wc.z, Tf = coef * wc.w;
         for (int ci = 0; ci < S; ++ci) {
-            const v4 wj = sW[p.sten[sell(c, ci, maxS)]];
+            const v4 wj = sW[sSt[sell(tid, ci, maxS)]];
             coef = a[(size_t)(ci + 1) * 64];
             pf += coef * wj.x;
             uxf += coef * wj.y;
@@ -268,6 +289,155 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
     p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
     smoothness = smoothness / ((R)0.5 * (pMax + pMin) * nt + K<R>::VSMALL);
     p.smooth[c] = smoothness > (R)1;
+}
+
+// K5 with four lanes per cell, for meshes whose cells have at most four
+// faces (triangles and quads): lane k of a cell's group handles face slot k
+// (its RBF sum and limiter candidates); the gradient and the compact min/max
+// are formed by every lane of the group.  The per-cell results are exact
+// reductions over the group (the limiter is a minimum, the smoothness a
+// maximum, the trouble test that of the last internal face), so the values
+// equal the one-lane-per-cell kernel's bit for bit.
+template <typename R, int TILE>
+__global__ __launch_bounds__(4 * TILE) void k_reconstruct_q(DevPtrs<R> p) {
+    using v4 = typename VT<R>::v4;
+    using v2 = typename VT<R>::v2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    v4* sW = reinterpret_cast<v4*>(smem_raw);
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= p.ntiles) return;
+    const int t = threadIdx.x;
+    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
+    uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
+    if (t < TILE && tile * TILE + t < p.nc) sW[t] = p.W[tile * TILE + t];
+    for (int i = t; i < hn; i += 4 * TILE) sW[TILE + i] = p.W[p.halo[hb + i]];
+    stage_stencil(sSt, p.sten, tile, TILE, p.nc, p.maxS, t, 4 * TILE);
+    __syncthreads();
+    const int lc = t >> 2, k = t & 3;
+    const int c = tile * TILE + lc;
+    if (c >= p.nc) return;  // the four lanes of a cell leave together
+    const int S = p.scnt[c], C = p.ccnt[c];
+    const int maxS = p.maxS, maxC = p.maxC, KR = maxC * (maxS + 1);
+    const v4 wc = sW[lc];
+
+    R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
+    for (int i = 0; i < C; ++i) {
+        const v2 M = p.lsq[sell(c, i, maxC)];
+        const v4 wj = sW[sSt[sell(lc, i, maxS)]];
+        R dphi = wj.x - wc.x;
+        gpx += M.x * dphi;
+        gpy += M.y * dphi;
+        dphi = wj.y - wc.y;
+        guxx += M.x * dphi;
+        guyx += M.y * dphi;
+        dphi = wj.z - wc.z;
+        guxy += M.x * dphi;
+        guyy += M.y * dphi;
+        dphi = wj.w - wc.w;
+        gTx += M.x * dphi;
+        gTy += M.y * dphi;
+    }
+    if (k == 0) p.gp[c] = v2{gpx, gpy};
+    if (k == 1) p.gu[c] = v2{guxx, guyx};
+    if (k == 2) p.gv[c] = v2{guxy, guyy};
+    if (k == 3) p.gT[c] = v2{gTx, gTy};
+
+    R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z, TMax = wc.w, TMin = wc.w;
+    for (int i = 0; i < C; ++i) {
+        const v4 wj = sW[sSt[sell(lc, i, maxS)]];
+        pMax = fmax(wj.x, pMax);
+        pMin = fmin(wj.x, pMin);
+        uxMax = fmax(wj.y, uxMax);
+        uxMin = fmin(wj.y, uxMin);
+        uyMax = fmax(wj.z, uyMax);
+        uyMin = fmin(wj.z, uyMin);
+        TMax = fmax(wj.w, TMax);
+        TMin = fmin(wj.w, TMin);
+    }
+
+    const int nf = p.nf;
+    const int sf = k < C ? p.slot_face[sell(c, k, maxC)] : 0;
+    const int isNei = sf < 0;
+    const int f = (sf < 0 ? -sf : sf) - 1;
+    const bool live = k < C && f < nf;
+    if (fabs(TMax - TMin) < K<R>::VSMALL) {
+        if (k == 0) {
+            p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
+            p.smooth[c] = 0;
+        }
+        if (live) {
+            const double2 d = p.slot_d[sell(c, k, maxC)];
+            const R dx = (R)d.x, dy = (R)d.y;
+            v4 o;
+            o.x = wc.x + gpx * dx + gpy * dy;
+            o.y = wc.y + guxx * dx + guyx * dy;
+            o.z = wc.z + guxy * dx + guyy * dy;
+            o.w = wc.w + gTx * dx + gTy * dy;
+            p.fv[2 * f + isNei] = o;
+            p.fvT[2 * f + isNei] = o.w;
+        }
+        return;
+    }
+
+    R limP = 2, limUx = 2, limUy = 2, limT = 2;
+    R smoothness = 0;
+    int faceNotTrouble = 1;
+    if (live) {
+        const R rP = fabs(pMax - pMin), rUx = fabs(uxMax - uxMin), rUy = fabs(uyMax - uyMin), rT = fabs(TMax - TMin);
+        const double2 d = p.slot_d[sell(c, k, maxC)];
+        const R dx = (R)d.x, dy = (R)d.y;
+        const double2 e = p.slot_e[sell(c, k, maxC)];
+        const R deltaR = (R)afar_hypot(e.x, e.y);
+        const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
+        const int nbl = sSt[sell(lc, k, maxS)];
+        const R* a = p.rbf + sell(c, k * (maxS + 1), KR);
+        R coef = a[0];
+        R pf = coef * wc.x, uxf = coef * wc.y, uyf = coef * wc.z, Tf = coef * wc.w;
+        for (int ci = 0; ci < S; ++ci) {
+            const v4 wj = sW[sSt[sell(lc, ci, maxS)]];
+            coef = a[(size_t)(ci + 1) * 64];
+            pf += coef * wj.x;
+            uxf += coef * wj.y;
+            uyf += coef * wj.z;
+            Tf += coef * wj.w;
+        }
+        v4 o;
+        R phif;
+        limiter_process(limP, phif, wc.x, pMax, pMin, gpx, gpy, dx, dy);
+        o.x = (R)(rP >= K<R>::SMALL) * pf + (R)(rP < K<R>::SMALL) * phif;
+        smoothness = fmax(smoothness, fabs(((R)1 - fraction) * wc.x + fraction * sW[nbl].x - phif) / deltaR +
+                                          (R)(pf <= (R)0) * (R)1e20 + (R)(phif <= (R)0) * (R)1e20);
+        limiter_process(limUx, phif, wc.y, uxMax, uxMin, guxx, guyx, dx, dy);
+        o.y = (R)(rUx >= K<R>::SMALL) * uxf + (R)(rUx < K<R>::SMALL) * phif;
+        limiter_process(limUy, phif, wc.z, uyMax, uyMin, guxy, guyy, dx, dy);
+        o.z = (R)(rUy >= K<R>::SMALL) * uyf + (R)(rUy < K<R>::SMALL) * phif;
+        limiter_process(limT, phif, wc.w, TMax, TMin, gTx, gTy, dx, dy);
+        o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
+        faceNotTrouble = phif > (R)0;
+        p.fv[2 * f + isNei] = o;
+        p.fvT[2 * f + isNei] = o.w;
+    }
+    // group reductions (lanes 4j..4j+3 of the wavefront)
+    for (int off = 1; off < 4; off <<= 1) {
+        limP = fmin(limP, (R)__shfl_xor(limP, off));
+        limUx = fmin(limUx, (R)__shfl_xor(limUx, off));
+        limUy = fmin(limUy, (R)__shfl_xor(limUy, off));
+        limT = fmin(limT, (R)__shfl_xor(limT, off));
+        smoothness = fmax(smoothness, (R)__shfl_xor(smoothness, off));
+    }
+    // the trouble test of the last internal face in slot order
+    const unsigned long long lm = __ballot(live);
+    const int base = (t & 63) & ~3;
+    const unsigned grp = (unsigned)(lm >> base) & 0xFu;
+    const int lastLane = grp ? 31 - __builtin_clz(grp) : 0;
+    const int nt0 = __shfl(faceNotTrouble, base + lastLane);
+    const int isNotTrouble = grp ? nt0 : 1;
+    if (k == 0) {
+        const R nt = (R)isNotTrouble;
+        p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
+        smoothness = smoothness / ((R)0.5 * (pMax + pMin) * nt + K<R>::VSMALL);
+        p.smooth[c] = smoothness > (R)1;
+    }
 }
 
 // ---------------------------------------------------------------- K6 BVD indicator
@@ -431,13 +601,17 @@ __device__ __forceinline__ void roe_flux(R& rhoFlux, R& rhoUxFlux, R& rhoUyFlux,
                           l3 * r3 * (hT + cT * VT_)) * magSf;
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double Cv) {
+// DT: also reduce the CFL bound of every internal face to one partial minimum
+// per block (dt_part[blockIdx.x]); the bound is a function of the
+// primitive state alone, which no kernel of the stage has changed yet.
+template <typename R, bool DT>
+__global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double Cv, double CFL) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int i = tile * 256 + threadIdx.x;
     const int nf = p.nf;
+    double dtf = 1e5;
     if (i < nf) {
         const int f = i;
         const int2 oc = p.fcell[f];
@@ -454,6 +628,10 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
         const v2 gpL = p.gp[own], guL = p.gu[own], gvL = p.gv[own], gTL = p.gT[own];
         const v2 gpR = p.gp[nei], guR = p.gu[nei], gvR = p.gv[nei], gTR = p.gT[nei];
         const v4 fL = p.fv[2 * f], fR = p.fv[2 * f + 1];
+        if (DT) {
+            const double d = cfl_face_dt(g0, g1, wL, wR, Rgas, Cv, CFL);
+            dtf = d < dtf ? d : dtf;
+        }
 
         const R pLT = wL.x + lL.x * (gpL.x * dxL + gpL.y * dyL);
         const R uxLT = wL.y + lL.y * (guL.x * dxL + guL.y * dyL);
@@ -504,6 +682,99 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
         p.flux[i] = double4{rhoFlux, rhoFlux * bUx + bp * g.x, rhoFlux * bUy + bp * g.y,
                             rhoFlux * (Cv * bT + 0.5 * (bUx * bUx + bUy * bUy) + bp / rho)};
     }
+    if (DT) {
+        // Two partials per tile of 256 faces: slot 0 for the instance of the
+        // tile's first face, slot 1 for the next one (every instance of a
+        // batch has at least 256 internal faces, so a tile meets at most two).
+        const int f0 = min(tile * 256, nf - 1);
+        const int i0 = (p.cinst && nf > 0) ? p.cinst[p.fcell[f0].x] : 0;
+        const int mine = (i < nf && p.cinst) ? p.cinst[p.fcell[i].x] : i0;
+        __shared__ double wmin[4];
+        const double m0 = block_min_256(mine == i0 ? dtf : 1e5, wmin);
+        const double m1 = block_min_256(mine == i0 ? 1e5 : dtf, wmin);
+        if (threadIdx.x == 0) {
+            p.dt_part[2 * tile] = m0;
+            p.dt_part[2 * tile + 1] = m1;
+        }
+    }
+}
+
+// Density residual norm: per block the sum over its cells of (Res_rho/V)^2
+// (the residual gathered as in the update), then one block adds the partials
+// in index order.  Fixed launch shapes make the sum order, and so the value,
+// reproducible.
+template <typename R>
+__global__ __launch_bounds__(256) void k_residual_part(DevPtrs<R> p) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (c < p.nc) {
+        const int C = p.ccnt[c], maxC = p.maxC;
+        double r0 = 0.0;
+        for (int fi = 0; fi < C; ++fi) {
+            const int sf = p.slot_face[sell(c, fi, maxC)];
+            const int dir = (sf > 0) - (sf < 0);
+            const int f = (sf < 0 ? -sf : sf) - 1;
+            r0 += dir * p.flux[f].x;
+        }
+        const double q = r0 / p.vol[c];
+        v = q * q;
+    }
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __shared__ double ws[4];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) p.res_part[blockIdx.x] = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+}
+
+__global__ __launch_bounds__(1024) void k_residual_sum(const double* part, int n, double* out) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += 1024) v += part[i];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __shared__ double ws[16];
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < 16; ++w) s += ws[w];
+        *out = s;
+    }
+}
+
+// Time step of instance blockIdx.x: minimum of the flux kernel's partials
+// over the tiles holding its internal faces (one block per instance).
+__global__ __launch_bounds__(1024) void k_dt_reduce(double* dt, const double* part, const int32_t* finst_off,
+                                                    double* tctl) {
+    const int inst = blockIdx.x;
+    const int fa = finst_off[inst], fb = finst_off[inst + 1];
+    double v = 1e5;
+    if (fb > fa) {
+        const int t0 = fa >> 8, t1 = (fb - 1) >> 8;
+        for (int t = t0 + (int)threadIdx.x; t <= t1; t += 1024) {
+            const double o = part[2 * t + ((t == t0 && (fa & 255)) ? 1 : 0)];
+            v = o < v ? o : v;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+    }
+    __shared__ double wmin[16];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = wmin[0];
+        for (int w = 1; w < 16; ++w) m = wmin[w] < m ? wmin[w] : m;
+        // clip to the end time as the reference's caller does with
+        // Foam::Time (dt = min(dt, t_end - t)); 0 once the end is reached
+        const double t = tctl[2 * inst], te = tctl[2 * inst + 1];
+        double d = 0.0;
+        if (t < te) {
+            const double rem = te - t;
+            d = m < rem ? m : rem;
+            tctl[2 * inst] = t + d;
+        }
+        dt[inst] = d;
+    }
 }
 
 // ---------------------------------------------------------------- K8-K11 update
@@ -531,7 +802,8 @@ __global__ __launch_bounds__(256) void k_update(DevPtrs<R> p, double Rgas, doubl
     const int c = tile * 256 + threadIdx.x;
     if (c >= p.nc) return;
     const int C = p.ccnt[c], maxC = p.maxC, nf = p.nf;
-    const double dt = dt_ptr ? *dt_ptr : dt_val;
+    const double dt = dt_ptr ? dt_ptr[p.cinst ? p.cinst[c] : 0] : dt_val;
+    if (dt_ptr && dt == 0.0) return;  // device path past its end time: the state stays
     double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
     for (int fi = 0; fi < C; ++fi) {
         const int sf = p.slot_face[sell(c, fi, maxC)];
@@ -594,8 +866,16 @@ void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream
 }
 
 template <typename R>
-void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s) {
+void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s, int lanes) {
     const int nb = pad8(p.ntiles);
+    if (lanes == 4) {
+        switch (tile) {
+            case 64: hipLaunchKernelGGL((k_reconstruct_q<R, 64>), dim3(nb), dim3(256), lds, s, p); break;
+            case 128: hipLaunchKernelGGL((k_reconstruct_q<R, 128>), dim3(nb), dim3(512), lds, s, p); break;
+            default: hipLaunchKernelGGL((k_reconstruct_q<R, 256>), dim3(nb), dim3(1024), lds, s, p); break;
+        }
+        return;
+    }
     switch (tile) {
         case 64: hipLaunchKernelGGL((k_reconstruct<R, 64>), dim3(nb), dim3(64), lds, s, p); break;
         case 128: hipLaunchKernelGGL((k_reconstruct<R, 128>), dim3(nb), dim3(128), lds, s, p); break;
@@ -613,10 +893,31 @@ void launch_bvd(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s) {
     }
 }
 
+int flux_blocks(int nf, int nb) { return pad8(std::max(1, (nf + nb + 255) / 256)); }
+
 template <typename R>
 void launch_flux(const DevPtrs<R>& p, double Rg, double Cv, hipStream_t s) {
-    const int nt = pad8(std::max(1, (p.nf + p.nb + 255) / 256));
-    hipLaunchKernelGGL(k_flux<R>, dim3(nt), dim3(256), 0, s, p, Rg, Cv);
+    hipLaunchKernelGGL((k_flux<R, false>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, 0.0);
+}
+
+template <typename R>
+void launch_dt_reduce(const DevPtrs<R>& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_dt_reduce, dim3(p.ninst), dim3(1024), 0, s, p.dt, (const double*)p.dt_part, p.finst_off,
+                       p.tctl);
+}
+
+int residual_blocks(int nc) { return std::max(1, (nc + 255) / 256); }
+
+template <typename R>
+void launch_residual_norm(const DevPtrs<R>& p, double* out, hipStream_t s) {
+    const int nb = residual_blocks(p.nc);
+    hipLaunchKernelGGL(k_residual_part<R>, dim3(nb), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(k_residual_sum, dim3(1), dim3(1024), 0, s, (const double*)p.res_part, nb, out);
+}
+
+template <typename R>
+void launch_flux_dt(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s) {
+    hipLaunchKernelGGL((k_flux<R, true>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, CFL);
 }
 
 template <typename R>
@@ -629,8 +930,9 @@ void launch_update(const DevPtrs<R>& p, double Rg, double gamma, double dt, cons
 template <typename R>
 hipError_t set_lds_limits(size_t lds) {
     hipError_t e = hipSuccess;
-    const void* fns[] = {(const void*)k_reconstruct<R, 64>, (const void*)k_reconstruct<R, 128>,
-                         (const void*)k_reconstruct<R, 256>};
+    const void* fns[] = {(const void*)k_reconstruct<R, 64>,   (const void*)k_reconstruct<R, 128>,
+                         (const void*)k_reconstruct<R, 256>,  (const void*)k_reconstruct_q<R, 64>,
+                         (const void*)k_reconstruct_q<R, 128>, (const void*)k_reconstruct_q<R, 256>};
     for (const void* fn : fns) {
         e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -641,9 +943,12 @@ hipError_t set_lds_limits(size_t lds) {
 #define AFAR_INSTANTIATE(R)                                                                                        \
     template void launch_lsq<R>(const DevPtrs<R>&, hipStream_t);                                                  \
     template void launch_cfl<R>(const DevPtrs<R>&, double, double, double, hipStream_t);                          \
-    template void launch_reconstruct<R>(const DevPtrs<R>&, int, size_t, hipStream_t);                             \
+    template void launch_reconstruct<R>(const DevPtrs<R>&, int, size_t, hipStream_t, int);                        \
     template void launch_bvd<R>(const DevPtrs<R>&, int, size_t, hipStream_t);                                     \
     template void launch_flux<R>(const DevPtrs<R>&, double, double, hipStream_t);                                 \
+    template void launch_flux_dt<R>(const DevPtrs<R>&, double, double, double, hipStream_t);                      \
+    template void launch_dt_reduce<R>(const DevPtrs<R>&, hipStream_t);                                            \
+    template void launch_residual_norm<R>(const DevPtrs<R>&, double*, hipStream_t);                               \
     template void launch_update<R>(const DevPtrs<R>&, double, double, double, const double*, double, double,      \
                                    double, int, hipStream_t);                                                     \
     template hipError_t set_lds_limits<R>(size_t);

@@ -110,6 +110,100 @@ RefArrays make_ref_arrays(const Mesh& m, const Precomputed& pc, const double* W0
     return ra;
 }
 
+RefArrays union_ref_arrays(const std::vector<const RefArrays*>& parts, BatchIndex& idx) {
+    const int n = (int)parts.size();
+    if (n == 0) throw Error(1, "empty batch");
+    idx.cell_off.assign(n + 1, 0);
+    idx.face_off.assign(n + 1, 0);
+    idx.bface_off.assign(n + 1, 0);
+    RefArrays u;
+    int64_t nc = 0, nf = 0, nb = 0;
+    for (int k = 0; k < n; ++k) {
+        const RefArrays& a = *parts[k];
+        nc += a.nc;
+        nf += a.nf;
+        nb += a.nb;
+        if (nc + nb > INT32_MAX / 2 || nf + nb > INT32_MAX / 2) throw Error(3, "batch too large for 32-bit indices");
+        idx.cell_off[k + 1] = (int32_t)nc;
+        idx.face_off[k + 1] = (int32_t)nf;
+        idx.bface_off[k + 1] = (int32_t)nb;
+        u.maxS = std::max(u.maxS, a.maxS);
+        u.maxC = std::max(u.maxC, a.maxC);
+        u.npatch += a.npatch;
+    }
+    u.nc = (int)nc;
+    u.nf = (int)nf;
+    u.nb = (int)nb;
+    const int maxS = u.maxS, maxC = u.maxC;
+    u.CELL.resize(2 * (size_t)(nc + nb));
+    u.FACE.resize(5 * (size_t)(nf + nb));
+    u.FD.resize(4 * (size_t)(nc + nb));
+    u.cellVolume.resize(nc);
+    u.cFD.resize(4 * (size_t)nc);
+    u.neighbour.resize(2 * (size_t)nf);
+    u.boundaryFaceNeiLabel.resize(nb);
+    u.boundaryFacesType.resize(nb);
+    u.stencilSize.resize(nc);
+    u.compactStencilSize.resize(nc);
+    u.stencil.assign((size_t)maxS * nc, 0);
+    u.cellFaces.assign((size_t)maxC * nc, 0);
+    u.RBFbasis.assign(2 * (size_t)(maxS + 1) * nf, 0.0);
+    u.FSHIFT.resize(2 * (size_t)nf);
+    u.BVALUE.resize(4 * (size_t)nb);
+    for (int k = 0; k < n; ++k) {
+        const RefArrays& a = *parts[k];
+        const int co = idx.cell_off[k], fo = idx.face_off[k], bo = idx.bface_off[k];
+        // cell id (real or pseudo) of part k -> batch id
+        auto cell = [&](int32_t id) -> int32_t { return id < a.nc ? co + id : (int32_t)nc + bo + (id - a.nc); };
+        // face id (internal or boundary) of part k -> batch id
+        auto face = [&](int32_t f) -> int32_t { return f < a.nf ? fo + f : (int32_t)nf + bo + (f - a.nf); };
+        for (int i = 0; i < a.nc + a.nb; ++i) {
+            const size_t d = cell(i);
+            u.CELL[2 * d] = a.CELL[2 * (size_t)i];
+            u.CELL[2 * d + 1] = a.CELL[2 * (size_t)i + 1];
+            for (int q = 0; q < 4; ++q) u.FD[4 * d + q] = a.FD[4 * (size_t)i + q];
+        }
+        for (int f = 0; f < a.nf + a.nb; ++f) {
+            const size_t d = face(f);
+            for (int q = 0; q < 5; ++q) u.FACE[5 * d + q] = a.FACE[5 * (size_t)f + q];
+        }
+        for (int c = 0; c < a.nc; ++c) {
+            const size_t d = co + c;
+            u.cellVolume[d] = a.cellVolume[c];
+            for (int q = 0; q < 4; ++q) u.cFD[4 * d + q] = a.cFD[4 * (size_t)c + q];
+            u.stencilSize[d] = a.stencilSize[c];
+            u.compactStencilSize[d] = a.compactStencilSize[c];
+            for (int i = 0; i < a.stencilSize[c]; ++i)
+                u.stencil[(size_t)maxS * d + i] = cell(a.stencil[(size_t)a.maxS * c + i]);
+            for (int i = 0; i < a.compactStencilSize[c]; ++i) {
+                const int32_t sf = a.cellFaces[(size_t)a.maxC * c + i];
+                const int32_t f = (sf < 0 ? -sf : sf) - 1;
+                const int32_t g = face(f) + 1;
+                u.cellFaces[(size_t)maxC * d + i] = sf < 0 ? -g : g;
+            }
+        }
+        for (int f = 0; f < a.nf; ++f) {
+            const size_t d = fo + f;
+            u.neighbour[2 * d] = cell(a.neighbour[2 * (size_t)f]);
+            u.neighbour[2 * d + 1] = cell(a.neighbour[2 * (size_t)f + 1]);
+            u.FSHIFT[2 * d] = a.FSHIFT[2 * (size_t)f];
+            u.FSHIFT[2 * d + 1] = a.FSHIFT[2 * (size_t)f + 1];
+            for (int side = 0; side < 2; ++side)
+                for (int i = 0; i <= a.maxS; ++i)
+                    u.RBFbasis[(2 * d + side) * (size_t)(maxS + 1) + i] =
+                        a.RBFbasis[(2 * (size_t)f + side) * (size_t)(a.maxS + 1) + i];
+        }
+        for (int b = 0; b < a.nb; ++b) {
+            const size_t d = bo + b;
+            u.boundaryFaceNeiLabel[d] = cell(a.boundaryFaceNeiLabel[b]);
+            u.boundaryFacesType[d] = a.boundaryFacesType[b];
+            for (int q = 0; q < 4; ++q) u.BVALUE[4 * d + q] = a.BVALUE[4 * (size_t)b + q];
+        }
+        u.boundaryFacesNum.insert(u.boundaryFacesNum.end(), a.boundaryFacesNum.begin(), a.boundaryFacesNum.end());
+    }
+    return u;
+}
+
 BlockTables make_block_tables(const RefArrays& ra) {
     const int B = 128, nc = ra.nc, maxS = ra.maxS;
     const int nblk = (nc + B - 1) / B;

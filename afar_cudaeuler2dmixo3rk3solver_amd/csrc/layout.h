@@ -74,6 +74,18 @@ struct DevLayout {
 
 DevLayout make_dev_layout(const RefArrays& ra, int tile);
 
+// A batch of independent meshes as one: the instances' cells, internal
+// faces, boundary faces (pseudo-cells) and patches each follow one another,
+// every index renumbered accordingly.  No stencil or face crosses instances,
+// so each cell computes exactly what it computes in its own mesh.
+struct BatchIndex {
+    std::vector<int32_t> cell_off;   // ninst+1, real cells
+    std::vector<int32_t> face_off;   // ninst+1, internal faces
+    std::vector<int32_t> bface_off;  // ninst+1, boundary faces
+    int ninst() const { return (int)cell_off.size() - 1; }
+};
+RefArrays union_ref_arrays(const std::vector<const RefArrays*>& parts, BatchIndex& idx);
+
 // The reference's 128-cell block tables (basicCUDAsolver.cpp:697-850): per
 // block the halo cells (compact-stencil halo first, then the rest of the
 // extended stencil) and block-local stencil ids.

@@ -156,8 +156,9 @@ PolyMesh generate_riemann(int n, double jitter, uint64_t seed, double split_p) {
     return pm;
 }
 
-PolyMesh generate_periodic(int n, double L, double jitter, uint64_t seed) {
+PolyMesh generate_periodic(int n, double L, double jitter, uint64_t seed, double split_p) {
     if (n < 3) throw Error(1, "periodic mesh needs n >= 3");
+    if (!(split_p >= 0.0 && split_p <= 1.0)) throw Error(1, "split_p must lie in [0, 1]");
     PolyMesh pm;
     pm.period = Vec2{L, L};
     pm.pts.resize((size_t)n * n);
@@ -179,6 +180,13 @@ PolyMesh generate_periodic(int n, double L, double jitter, uint64_t seed) {
     };
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
+            // the split draw is skipped when every quad is split, so the
+            // all-triangle mesh of a seed does not depend on split_p
+            const bool split = split_p >= 1.0 || rng.uniform() < split_p;
+            if (!split) {
+                add({{i, j}, {i + 1, j}, {i + 1, j + 1}, {i, j + 1}});
+                continue;
+            }
             const bool diag = rng.uniform() < 0.5;
             if (!diag) {
                 add({{i, j}, {i + 1, j}, {i + 1, j + 1}});

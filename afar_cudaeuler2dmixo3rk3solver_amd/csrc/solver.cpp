@@ -2,6 +2,7 @@
 // (src/CUDA/cudaEuler2DmixO3RK3/GPUevolve.cu:177-306) and graph capture.
 #include "solver.h"
 
+#include <math.h>
 #include <string.h>
 
 #include <cstdio>
@@ -38,13 +39,26 @@ class SolverImpl final : public DeviceSolver {
     using v2 = typename VT<R>::v2;
 
    public:
-    SolverImpl(const RefArrays& ra, const SolverConfig& cfg) : ra_(ra) {
+    SolverImpl(const RefArrays& ra, const SolverConfig& cfg, const BatchIndex* bi) : ra_(ra) {
         cfg_ = cfg;
+        if (bi) {
+            bi_ = *bi;
+            for (int k = 0; k < bi_.ninst(); ++k)
+                if (bi_.face_off[k + 1] - bi_.face_off[k] < 256)
+                    throw Error(1, "every mesh of a batch needs at least 256 internal faces");
+        } else {
+            bi_.cell_off = {0, ra.nc};
+            bi_.face_off = {0, ra.nf};
+            bi_.bface_off = {0, ra.nb};
+        }
         L_ = make_dev_layout(ra, cfg.tile);
         check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
-        lds_recon_ = (size_t)(L_.tile + L_.maxHalo) * sizeof(v4);
+        // LDS of the reconstruction: tile + halo states, then the tile's stencil ids
+        lds_recon_ = (size_t)(L_.tile + L_.maxHalo) * sizeof(v4) + (size_t)L_.tile * L_.maxS * sizeof(uint16_t);
+        if (cfg.reconLanes == 4 && L_.maxC > 4) throw Error(1, "reconLanes 4 needs cells with at most four faces");
+        lanes_ = cfg.reconLanes == 1 ? 1 : (L_.maxC <= 4 ? 4 : 1);
         lds_bvd_ = (size_t)(L_.tile + L_.maxHalo);
         int dev = 0;
         check_hip(hipGetDevice(&dev), "hipGetDevice");
@@ -69,6 +83,7 @@ class SolverImpl final : public DeviceSolver {
     }
 
     double compute_dt() override {
+        if (ninst() > 1) throw Error(1, "a batch steps with the device time step only");
         launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
         check_hip(hipGetLastError(), "k_cfl");
         double dt = 0.0;
@@ -78,6 +93,7 @@ class SolverImpl final : public DeviceSolver {
     }
 
     void evolve(double dt) override {
+        if (ninst() > 1) throw Error(1, "a batch steps with the device time step only");
         enqueue_stages(dt, nullptr);
         check_hip(hipGetLastError(), "evolve");
     }
@@ -88,6 +104,13 @@ class SolverImpl final : public DeviceSolver {
     }
 
     void synchronize() override { check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+    int ninst() const override { return bi_.ninst(); }
+
+    void last_dts(double* out) override {
+        check_hip(hipMemcpyAsync(out, P_.dt, sizeof(double) * ninst(), hipMemcpyDeviceToHost, stream_), "dt copy");
+        synchronize();
+    }
 
     double last_dt() override {
         double dt = 0.0;
@@ -123,6 +146,67 @@ class SolverImpl final : public DeviceSolver {
         synchronize();
     }
 
+    void get_conserved_range(int c0, int n, double* U) override {
+        if (c0 < 0 || n < 0 || c0 + n > L_.nc) throw Error(1, "cell range out of bounds");
+        check_hip(hipMemcpyAsync(U, P_.U + c0, sizeof(double4) * n, hipMemcpyDeviceToHost, stream_), "U copy");
+        synchronize();
+    }
+
+    void get_primitive_range(int c0, int n, double* W, int8_t* type) override {
+        if (c0 < 0 || n < 0 || c0 + n > L_.nc) throw Error(1, "cell range out of bounds");
+        std::vector<v4> w(W ? n : 0);
+        if (W) check_hip(hipMemcpyAsync(w.data(), P_.W + c0, sizeof(v4) * n, hipMemcpyDeviceToHost, stream_), "W copy");
+        if (type) check_hip(hipMemcpyAsync(type, P_.ftype + c0, n, hipMemcpyDeviceToHost, stream_), "type copy");
+        synchronize();
+        for (int i = 0; i < (W ? n : 0); ++i) {
+            W[4 * (size_t)i] = w[i].x;
+            W[4 * (size_t)i + 1] = w[i].y;
+            W[4 * (size_t)i + 2] = w[i].z;
+            W[4 * (size_t)i + 3] = w[i].w;
+        }
+    }
+
+    void reset_state() override { upload_state(); }
+
+    void set_time(int inst, double t, double t_end) override {
+        if (inst < -1 || inst >= ninst()) throw Error(1, "instance out of range");
+        if (!(t_end >= t)) throw Error(1, "end time before the current time");
+        std::vector<double> tc(2 * (size_t)ninst());
+        check_hip(hipMemcpyAsync(tc.data(), P_.tctl, sizeof(double) * tc.size(), hipMemcpyDeviceToHost, stream_),
+                  "clock copy");
+        synchronize();
+        for (int k = 0; k < ninst(); ++k)
+            if (inst < 0 || inst == k) {
+                tc[2 * k] = t;
+                tc[2 * k + 1] = t_end;
+            }
+        check_hip(hipMemcpyAsync(P_.tctl, tc.data(), sizeof(double) * tc.size(), hipMemcpyHostToDevice, stream_),
+                  "clock copy");
+        synchronize();
+    }
+
+    void get_time(double* t) override {
+        std::vector<double> tc(2 * (size_t)ninst());
+        check_hip(hipMemcpyAsync(tc.data(), P_.tctl, sizeof(double) * tc.size(), hipMemcpyDeviceToHost, stream_),
+                  "clock copy");
+        synchronize();
+        for (int k = 0; k < ninst(); ++k) t[k] = tc[2 * k];
+    }
+
+    double residual_norm() override {
+        launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_, lanes_);
+        launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
+        launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
+        launch_residual_norm<R>(P_, P_.res_part + residual_blocks(L_.nc), stream_);
+        check_hip(hipGetLastError(), "residual");
+        double s = 0.0;
+        check_hip(hipMemcpyAsync(&s, P_.res_part + residual_blocks(L_.nc), sizeof(double), hipMemcpyDeviceToHost,
+                                 stream_),
+                  "residual copy");
+        synchronize();
+        return sqrt(s / L_.nc);
+    }
+
     void set_conserved(const double* U) override {
         // primitive state from the conserved one, as the update kernel does it
         const double gamma = cfg_.R / cfg_.Cv + 1.0;
@@ -146,10 +230,7 @@ class SolverImpl final : public DeviceSolver {
         if (graph) {
             for (int i = 0; i < nsteps; ++i) check_hip(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
         } else {
-            for (int i = 0; i < nsteps; ++i) {
-                launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
-                enqueue_stages(0.0, P_.dt);
-            }
+            for (int i = 0; i < nsteps; ++i) enqueue_stages(0.0, P_.dt);
         }
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
@@ -168,9 +249,11 @@ class SolverImpl final : public DeviceSolver {
             for (int r = 0; r < reps; ++r) {
                 switch (k) {
                     case kKernCfl: launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_); break;
-                    case kKernRecon: launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_); break;
+                    case kKernRecon: launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_, lanes_); break;
                     case kKernBvd: launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_); break;
                     case kKernFlux: launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_); break;
+                    case kKernFluxDt: launch_flux_dt<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_); break;
+                    case kKernDtReduce: launch_dt_reduce<R>(P_, stream_); break;
                     default: launch_update<R>(P_, cfg_.R, gamma(), 0.0, P_.dt, b1(1), b2(1), b3(1), 0, stream_);
                 }
             }
@@ -179,7 +262,13 @@ class SolverImpl final : public DeviceSolver {
             float t = 0.f;
             check_hip(hipEventElapsedTime(&t, ev0_, ev1_), "hipEventElapsedTime");
             ms[k] = t / reps;
-            count[k] = k == kKernCfl ? 1 : 3;  // launches per RK3 step
+        }
+        // launches per RK3 step with the device-side time step
+        count[kKernCfl] = 0;
+        count[kKernRecon] = count[kKernBvd] = count[kKernUpdate] = 3;
+        count[kKernFlux] = 2;
+        count[kKernFluxDt] = count[kKernDtReduce] = 1;
+        for (int k = 0; k < kKernKinds; ++k) {
         }
     }
 
@@ -226,7 +315,7 @@ class SolverImpl final : public DeviceSolver {
     }
 
     void run_stage_kernels(int upto, double dt, int stage) override {
-        launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_);
+        launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_, lanes_);
         if (upto > 0) launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
         if (upto > 1) launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
         if (upto > 2) launch_update<R>(P_, cfg_.R, gamma(), dt, nullptr, b1(stage), b2(stage), b3(stage), stage == 2, stream_);
@@ -250,11 +339,20 @@ class SolverImpl final : public DeviceSolver {
         return cfg_.exactRK ? ex[s] : lit[s];
     }
 
+    // One RK3 step.  dt_ptr == nullptr: the host's dt.  Otherwise the step's
+    // time step is evaluated on the device by the first flux pass (from the
+    // state the step starts from, exactly as the standalone CFL pass would)
+    // and read by the updates from dt_ptr.
     void enqueue_stages(double dt, const double* dt_ptr) {
         for (int s = 0; s < 3; ++s) {
-            launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_);
+            launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_, lanes_);
             launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
-            launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
+            if (s == 0 && dt_ptr) {
+                launch_flux_dt<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
+                launch_dt_reduce<R>(P_, stream_);
+            } else {
+                launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
+            }
             launch_update<R>(P_, cfg_.R, gamma(), dt, dt_ptr, b1(s), b2(s), b3(s), s == 2, stream_);
         }
     }
@@ -262,7 +360,6 @@ class SolverImpl final : public DeviceSolver {
     void capture() {
         synchronize();
         check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        launch_cfl<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
         enqueue_stages(0.0, P_.dt);
         check_hip(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
         check_hip(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -298,6 +395,7 @@ class SolverImpl final : public DeviceSolver {
         P_.maxS = L_.maxS;
         P_.maxC = L_.maxC;
         P_.ntiles = L_.ntiles;
+        P_.lds_sten_off = (int)((size_t)(L_.tile + L_.maxHalo) * sizeof(v4));
         P_.boundaryFirstOrder = cfg_.boundaryFirstOrder;
         m_W.alloc(sizeof(v4) * (nc + nb));
         P_.W = m_W.as<v4>();
@@ -346,15 +444,41 @@ class SolverImpl final : public DeviceSolver {
         P_.blabel = put(m_blabel, L_.blabel.data(), nb);
         P_.bgeo = reinterpret_cast<const double4*>(put(m_bgeo, L_.bgeo.data(), 4 * nb));
         P_.bval = reinterpret_cast<const double4*>(put(m_bval, L_.bval.data(), 4 * nb));
-        // dt, the ticket counter (own 256-byte line) and the CFL partials
-        m_dt.alloc(512 + sizeof(double) * kCflBlocks);
-        P_.dt = m_dt.as<double>();
-        P_.dt_count = reinterpret_cast<unsigned int*>(m_dt.as<char>() + 256);
-        P_.dt_part = reinterpret_cast<double*>(m_dt.as<char>() + 512);
+        // per-instance time steps, the ticket counter (own 256-byte line) and
+        // the CFL partials (two per flux tile, or kCflBlocks for the
+        // standalone pass)
+        const int ni = bi_.ninst();
+        P_.ninst = ni;
+        if (ni > 1) {
+            std::vector<int32_t> ci(nc);
+            for (int k = 0; k < ni; ++k)
+                for (int c = bi_.cell_off[k]; c < bi_.cell_off[k + 1]; ++c) ci[c] = k;
+            P_.cinst = put(m_cinst, ci.data(), nc);
+        } else {
+            P_.cinst = nullptr;
+        }
+        P_.finst_off = put(m_finst, bi_.face_off.data(), ni + 1);
         {
-            const double init_dt = 1e5;
+            std::vector<double> tc(2 * (size_t)ni);
+            for (int k = 0; k < ni; ++k) {
+                tc[2 * k] = 0.0;
+                tc[2 * k + 1] = HUGE_VAL;
+            }
+            P_.tctl = put(m_tctl, tc.data(), tc.size());
+        }
+        m_res.alloc(sizeof(double) * (residual_blocks(L_.nc) + 1));
+        P_.res_part = m_res.as<double>();
+        const int dt_cap = std::max(kCflBlocks, 2 * flux_blocks(L_.nf, L_.nb));
+        const size_t dt_bytes = (sizeof(double) * ni + 255) / 256 * 256;
+        m_dt.alloc(dt_bytes + 256 + sizeof(double) * dt_cap);
+        P_.dt_cap = dt_cap;
+        P_.dt = m_dt.as<double>();
+        P_.dt_count = reinterpret_cast<unsigned int*>(m_dt.as<char>() + dt_bytes);
+        P_.dt_part = reinterpret_cast<double*>(m_dt.as<char>() + dt_bytes + 256);
+        {
+            const std::vector<double> init_dt(ni, 1e5);
             unsigned int zero = 0;
-            check_hip(hipMemcpy(P_.dt, &init_dt, 8, hipMemcpyHostToDevice), "dt init");
+            check_hip(hipMemcpy(P_.dt, init_dt.data(), sizeof(double) * ni, hipMemcpyHostToDevice), "dt init");
             check_hip(hipMemcpy(P_.dt_count, &zero, 4, hipMemcpyHostToDevice), "dt init");
         }
         upload_state();
@@ -365,6 +489,7 @@ class SolverImpl final : public DeviceSolver {
     }
 
     RefArrays ra_;
+    BatchIndex bi_;
     DevLayout L_;
     DevPtrs<R> P_{};
     hipStream_t stream_ = nullptr;
@@ -372,17 +497,19 @@ class SolverImpl final : public DeviceSolver {
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     size_t lds_recon_ = 0, lds_bvd_ = 0;
+    int lanes_ = 1;  // threads per cell of the reconstruction kernel
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fv, m_flux, m_bcell,
-        m_blabel, m_bgeo, m_bval, m_dt, m_fvT;
+        m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res;
 };
 
 }  // namespace
 
-std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg) {
+std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg,
+                                                 const BatchIndex* bi) {
     if (ra.nc <= 0) throw Error(1, "empty mesh");
-    if (cfg.precision == 1) return std::unique_ptr<DeviceSolver>(new SolverImpl<float>(ra, cfg));
-    return std::unique_ptr<DeviceSolver>(new SolverImpl<double>(ra, cfg));
+    if (cfg.precision == 1) return std::unique_ptr<DeviceSolver>(new SolverImpl<float>(ra, cfg, bi));
+    return std::unique_ptr<DeviceSolver>(new SolverImpl<double>(ra, cfg, bi));
 }
 
 }  // namespace afar

@@ -17,10 +17,23 @@ struct SolverConfig {
     int exactRK = 0;
     int tile = 256;
     int boundaryFirstOrder = 0;
+    int reconLanes = 0;  // reconstruction threads per cell: 0 automatic, 1, or 4
 };
 
 // kernel kinds reported by profile()
-enum { kKernCfl = 0, kKernRecon = 1, kKernBvd = 2, kKernFlux = 3, kKernUpdate = 4, kKernKinds = 5 };
+// Kernel kinds of profile(): the standalone CFL pass (host-requested time
+// steps), the four stage kernels, the flux pass that also reduces the CFL
+// bound (first stage of a device-time-step RK3 step) and its final reduction.
+enum {
+    kKernCfl = 0,
+    kKernRecon = 1,
+    kKernBvd = 2,
+    kKernFlux = 3,
+    kKernUpdate = 4,
+    kKernFluxDt = 5,
+    kKernDtReduce = 6,
+    kKernKinds = 7
+};
 
 class DeviceSolver {
    public:
@@ -33,6 +46,24 @@ class DeviceSolver {
     virtual void get_conserved(double* U) = 0;          // 4*nc
     virtual void set_conserved(const double* U) = 0;    // also recomputes W and ghosts
     virtual double last_dt() = 0;
+    // Instances of a batch (1 for one mesh) and the time step of each.
+    virtual int ninst() const = 0;
+    virtual void last_dts(double* out) = 0;
+    // Cells [c0, c0+n): conserved state (4n); primitive state (4n) and final
+    // reconstruction types (n); either output may be NULL.
+    virtual void get_conserved_range(int c0, int n, double* U) = 0;
+    virtual void get_primitive_range(int c0, int n, double* W, int8_t* type) = 0;
+    // Back to the state the solver was created with (or last given to
+    // set_conserved); the device clock is not changed.
+    virtual void reset_state() = 0;
+    // Device clock of the device-time-step path: instance inst (-1: all) is
+    // at time t and stops at t_end (steps past it leave the state alone).
+    virtual void set_time(int inst, double t, double t_end) = 0;
+    virtual void get_time(double* t) = 0;  // ninst values
+    // Root mean square over cells of the density residual per unit volume,
+    // |sum_f F_rho| / V, of the current state (runs the stage kernels, so the
+    // switch flags become those of the current state).
+    virtual double residual_norm() = 0;
     virtual double time_steps(int nsteps, bool graph) = 0;  // hipEvent milliseconds for nsteps
     virtual void profile(double* ms, int* count, int reps) = 0;
     virtual const DevLayout& layout() const = 0;
@@ -47,7 +78,10 @@ class DeviceSolver {
     SolverConfig cfg_;
 };
 
-std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg);
+// bi: instance offsets when ra is a batch (union_ref_arrays), else nullptr.
+// A batch steps with the device time step only (evolve_auto, time_steps).
+std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg,
+                                                 const BatchIndex* bi = nullptr);
 void check_hip(hipError_t e, const char* what);
 
 }  // namespace afar

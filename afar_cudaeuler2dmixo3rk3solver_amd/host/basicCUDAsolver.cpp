@@ -86,7 +86,7 @@ caseMesh::caseMesh(const std::string& caseDir) {
               "afar_mesh_generate_riemann");
     } else if (type == "periodic") {
         check(afar_mesh_generate_periodic((int32_t)m.lookupLabel("n"), m.lookupOrDefault("L", 10.0), jitter,
-                                          (uint64_t)seed, &mesh_),
+                                          (uint64_t)seed, m.lookupOrDefault("splitP", 1.0), &mesh_),
               "afar_mesh_generate_periodic");
     } else {
         throw solverError(AFAR_ERR_INVALID, "unknown mesh type " + type);

@@ -19,9 +19,9 @@ STENCIL_TYPES = {"faceNeighbour": 0, "edgeNeighbour": 1}
 
 def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilType="edgeNeighbour",
             recalcBasis=True, cOrder=5, fOrder=4, precision="fp64", exactRK=False, tile=256,
-            boundaryFirstOrder=False):
+            boundaryFirstOrder=False, reconLanes=0):
     """afar_schemes with the reference's key names and defaults (the last
-    four are this library's extensions)."""
+    five are this library's extensions)."""
     s = Schemes()
     lib().afar_schemes_default(C.byref(s))
     s.R, s.Cv, s.maxCo, s.shapeFactor = R, Cv, maxCo, shapeFactor
@@ -32,6 +32,7 @@ def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilTy
     s.exactRK = int(bool(exactRK))
     s.tile = tile
     s.boundaryFirstOrder = int(bool(boundaryFirstOrder))
+    s.reconLanes = int(reconLanes)
     return s
 
 
@@ -69,9 +70,9 @@ class Mesh:
         return cls(h)
 
     @classmethod
-    def periodic(cls, n=64, L=10.0, jitter=0.15, seed=0):
+    def periodic(cls, n=64, L=10.0, jitter=0.15, seed=0, split_p=1.0):
         h = C.c_void_p()
-        check(lib().afar_mesh_generate_periodic(n, L, jitter, seed, C.byref(h)))
+        check(lib().afar_mesh_generate_periodic(n, L, jitter, seed, split_p, C.byref(h)))
         return cls(h)
 
     @classmethod
@@ -257,6 +258,34 @@ class BasicCUDASolver:
     def synchronize(self):
         check(lib().afar_solver_synchronize(self._h))
 
+    def set_time(self, t, t_end=float("inf")):
+        check(lib().afar_solver_set_time(self._h, t, t_end))
+
+    def device_time(self):
+        t = C.c_double()
+        check(lib().afar_solver_get_time(self._h, C.byref(t)))
+        return t.value
+
+    def run_until(self, t_end, chunk=64, max_steps=None):
+        """Device-clocked steps from the current device time to t_end (the
+        last step clipped), `chunk` graph launches between clock reads.
+        Returns the number of graph launches (the last chunk may hold
+        steps past t_end, which change nothing)."""
+        t = self.device_time()
+        self.set_time(t, t_end)
+        n = 0
+        while t < t_end and (max_steps is None or n < max_steps):
+            k = chunk if max_steps is None else min(chunk, max_steps - n)
+            self.evolve_auto(k)
+            n += k
+            t = self.device_time()
+        return n
+
+    def residual_norm(self):
+        r = C.c_double()
+        check(lib().afar_solver_residual_norm(self._h, C.byref(r)))
+        return r.value
+
     def write_field(self):
         W = np.zeros((self.nc, 4))
         g = np.zeros((self.nb, 4))
@@ -279,10 +308,11 @@ class BasicCUDASolver:
         return ms.value
 
     def profile(self, reps=1):
-        ms = np.zeros(5)
-        cnt = np.zeros(5, dtype=np.int32)
+        names = ["cfl", "reconstruct", "bvd", "flux", "update", "flux_dt", "dt_reduce"]
+        ms = np.zeros(len(names))
+        cnt = np.zeros(len(names), dtype=np.int32)
         check(lib().afar_solver_profile(self._h, reps, dptr(ms), ptr(cnt, C.c_int32)))
-        return dict(zip(["cfl", "reconstruct", "bvd", "flux", "update"], zip(ms.tolist(), cnt.tolist())))
+        return dict(zip(names, zip(ms.tolist(), cnt.tolist())))
 
     def device_sizes(self):
         o = np.zeros(8, dtype=np.int64)

@@ -61,6 +61,8 @@ typedef struct afar_schemes {
     int32_t boundaryFirstOrder; /* extension, 0 = reference behaviour: cells with a boundary face
                                    use first-order face states (the reference does this only for
                                    cells with two boundary faces, cudaLimiter.h:419-429) */
+    int32_t reconLanes;  /* threads per cell of the reconstruction kernel: 0 automatic (4 when
+                            every cell has at most four faces), 1, or 4; same results */
 } afar_schemes;
 
 const char* afar_last_error(void);
@@ -93,8 +95,9 @@ int afar_mesh_generate_corner(int32_t nx, int32_t ny, double lx, double ly, doub
  * triangles with probability split_p (diagonal chosen by a second draw).
  * Four patches (left, right, bottom, top), all label 4. */
 int afar_mesh_generate_riemann(int32_t n, double jitter, uint64_t seed, double split_p, afar_mesh** out);
-/* Doubly periodic [0,L]^2, n x n quads split into 2n^2 triangles, all nodes jittered. */
-int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t seed, afar_mesh** out);
+/* Doubly periodic [0,L]^2, n x n quads, all nodes jittered; each quad split
+ * into two triangles with probability split_p (1: 2n^2 triangles). */
+int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t seed, double split_p, afar_mesh** out);
 void afar_mesh_destroy(afar_mesh* m);
 /* Sets the boundary label of the patch-th non-empty patch, as the reference
  * reads CUDA.boundary.patch<i> (basicCUDAsolver.cpp:170-172); value (p, Ux,
@@ -148,6 +151,15 @@ int afar_solver_evolve(afar_solver* h, double dt);
 int afar_solver_evolve_auto(afar_solver* h, int32_t nsteps);
 /* Time step used by the last evolve_auto step (synchronises). */
 int afar_solver_last_dt(afar_solver* h, double* dt);
+/* Device clock of evolve_auto: the solver is at time t and its device time
+ * steps are clipped to end exactly at t_end (dt = min(dt, t_end - t), as the
+ * reference application's Foam::Time does); steps past t_end leave the state
+ * unchanged.  Initially t = 0, t_end = +inf. */
+int afar_solver_set_time(afar_solver* h, double t, double t_end);
+int afar_solver_get_time(afar_solver* h, double* t);
+/* sqrt(mean over cells of (sum_f F_rho / V)^2) for the current state (runs
+ * the stage kernels; the switch flags become those of the current state). */
+int afar_solver_residual_norm(afar_solver* h, double* norm);
 int afar_solver_synchronize(afar_solver* h);
 /* basicCUDAsolver::writeField (basicCUDAsolver.cpp:874-913): W 4*nc and
  * shock nc (final reconstruction type, shockIndicator[2c+1]) in the caller's
@@ -159,8 +171,10 @@ int afar_solver_get_conserved(afar_solver* h, double* U);
 int afar_solver_set_conserved(afar_solver* h, const double* U);
 /* hipEvent time of nsteps auto-dt steps on the solver's stream (graph replay if use_graph). */
 int afar_solver_time_steps(afar_solver* h, int32_t nsteps, int32_t use_graph, double* ms);
-/* Per-kernel event times of reps auto-dt steps: ms[5], count[5] for
- * {cfl, reconstruct, bvd, flux, update}. */
+/* Average time of `reps` back-to-back launches of each kernel: ms[7] and
+ * launches per device-time-step RK3 step count[7] for {cfl (standalone, used
+ * by set_delta_t), reconstruct, bvd, flux, update, flux with CFL partials,
+ * CFL partial reduction}. */
 int afar_solver_profile(afar_solver* h, int32_t reps, double* ms, int32_t* count);
 /* Device sizes: out[0..7] = nc, nf, nb, maxS, maxC, tile, ntiles, maxHalo. */
 int afar_solver_sizes(afar_solver* h, int64_t* out);
@@ -169,6 +183,42 @@ int afar_solver_sizes(afar_solver* h, int64_t* out);
 int afar_solver_run_stage(afar_solver* h, int32_t upto, double dt, int32_t stage);
 int afar_solver_fetch(afar_solver* h, const char* name, void* out, size_t bytes);
 void afar_solver_destroy(afar_solver* h);
+
+/* ------------------------------------------------------------------ batches
+ * Extension without a reference counterpart (the reference solves one mesh
+ * per process): n independent precomputed meshes advanced together.  The
+ * meshes are stored one after another in one device layout, so every kernel
+ * launch of a step covers all of them and one graph launch advances the whole
+ * batch by one RK3 step; each mesh keeps its own CFL time step.  Each mesh
+ * gets exactly the trajectory it gets alone (same arithmetic per cell).  One
+ * schemes (gas constants, CFL, precision, tile) for all; W0[k] is the initial
+ * state of mesh k in its caller order.  Every mesh needs >= 256 internal faces. */
+typedef struct afar_batch afar_batch;
+int afar_batch_create(int32_t n, const afar_mesh* const* meshes, const afar_schemes* s, const double* const* W0,
+                      afar_batch** out);
+/* nsteps RK3 steps of every mesh, device time steps, asynchronous. */
+int afar_batch_evolve(afar_batch* b, int32_t nsteps);
+/* Same, synchronous, hipEvent milliseconds in *ms. */
+int afar_batch_time_steps(afar_batch* b, int32_t nsteps, double* ms);
+int afar_batch_synchronize(afar_batch* b);
+/* Back to the initial states. */
+int afar_batch_reset(afar_batch* b);
+/* out[0..7] = meshes, cells, internal faces, boundary faces, tile, maxS, maxC, maxHalo. */
+int afar_batch_sizes(afar_batch* b, int64_t* out);
+/* off[n+1]: first cell of each mesh in the batch layout. */
+int afar_batch_cell_offsets(afar_batch* b, int32_t* off);
+/* dt[n]: the time step each mesh took in its last step. */
+int afar_batch_last_dt(afar_batch* b, double* dt);
+/* Device clocks as afar_solver_set_time, for mesh inst (-1: all); t[n]. */
+int afar_batch_set_time(afar_batch* b, int32_t inst, double t, double t_end);
+int afar_batch_get_time(afar_batch* b, double* t);
+/* Conserved state (4 per cell) of mesh inst, its caller order. */
+int afar_batch_get_conserved(afar_batch* b, int32_t inst, double* U);
+/* Primitive state and final reconstruction types of mesh inst; either may be NULL. */
+int afar_batch_write_field(afar_batch* b, int32_t inst, double* W, int8_t* shock);
+/* Per-kernel times as afar_solver_profile. */
+int afar_batch_profile(afar_batch* b, int32_t reps, double* ms, int32_t* count);
+void afar_batch_destroy(afar_batch* b);
 
 /* Reference-layout arrays of a precomputed mesh (what seam 2 receives), for
  * checking against a host implementation.  The pointers in `out` must be

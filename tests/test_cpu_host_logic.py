@@ -1,0 +1,54 @@
+"""Host logic of the batched and served solves (no GPU): request streams,
+size buckets, instance parameters, footprint estimates."""
+import math
+
+import numpy as np
+
+
+def test_request_stream_statistics():
+    from afar_cudaeuler2dmixo3rk3solver_amd.serve import request_stream
+    reqs = request_stream(rate=20.0, duration=120.0, seed=3)
+    assert reqs == request_stream(rate=20.0, duration=120.0, seed=3)  # deterministic
+    n = len(reqs)
+    assert abs(n - 2400) < 4 * math.sqrt(2400)  # Poisson count
+    t = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(t) > 0) and t[-1] < 120.0
+    kinds = np.array([r.kind == "corner" for r in reqs])
+    assert abs(kinds.mean() - 0.5) < 0.05
+    nc = np.array([r.nc_target for r in reqs])
+    assert nc.min() >= 1e4 and nc.max() <= 2e6
+    # log-uniform: log10 sizes spread evenly over [4, 6.3]
+    lg = np.log10(nc)
+    assert abs(lg.mean() - (4 + math.log10(2e6)) / 2) < 0.05
+
+
+def test_size_buckets_and_instances():
+    from afar_cudaeuler2dmixo3rk3solver_amd.batch import corner_instances, size_buckets
+    ps = corner_instances(n=256, seed=2)
+    assert ps == corner_instances(n=256, seed=2)
+    cells = np.array([p["nx"] * p["ny"] for p in ps])
+    assert cells.min() >= 4500 and cells.max() <= 2.1e5
+    assert abs(cells.mean() - 52860) < 0.15 * 52860
+    for p in ps:
+        assert 1.5 <= p["mach_u"] <= 4.0 and 5.0 <= p["angle_deg"] <= 25.0
+        assert 1.5 < p["nx"] / p["ny"] < 2.6
+    b = size_buckets(cells, per_decade=2)
+    assert sorted(i for ids in b.values() for i in ids) == list(range(256))
+    for k, ids in b.items():
+        lg = np.log10(cells[ids]) * 2
+        assert np.all(np.floor(lg) == k)
+
+
+def test_footprint_grows_with_mesh(afar):
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    from afar_cudaeuler2dmixo3rk3solver_amd.serve import footprint_bytes
+    sc = afar.schemes(cOrder=7, fOrder=2)
+    prev = 0
+    for nx in (20, 40, 80):
+        m, _ = cases.corner_case(nx=nx, ny=nx // 2)
+        m.precompute(sc)
+        f64, f32 = footprint_bytes(m, "fp64"), footprint_bytes(m, "mixed")
+        assert f64 > f32 > prev
+        # at least the state, coefficients and face arrays: > 400 B per cell in FP64
+        assert f64 > 400 * m.nc
+        prev = f32

@@ -243,3 +243,45 @@ def test_boundary_face_rules(corner, order, npts, deg):
         for k in range(deg + 1):
             ref = 0.0 if k % 2 else half ** k / (k + 1)
             assert abs((w * t ** k).sum() - ref) < 1e-14
+
+
+def test_periodic_split_option(afar):
+    tri = afar.Mesh.periodic(n=8, L=10.0, jitter=0.15, seed=3)
+    tri1 = afar.Mesh.periodic(n=8, L=10.0, jitter=0.15, seed=3, split_p=1.0)
+    quad = afar.Mesh.periodic(n=8, L=10.0, jitter=0.15, seed=3, split_p=0.0)
+    mix = afar.Mesh.periodic(n=8, L=10.0, jitter=0.15, seed=3, split_p=0.5)
+    assert tri.nc == 128 and quad.nc == 64 and 64 < mix.nc < 128
+    assert np.array_equal(tri.cells()[0], tri1.cells()[0])
+    for m in (tri, quad, mix):
+        _, vol = m.cells()
+        assert abs(vol.sum() - 100.0) < 1e-9  # the periodic square, unit depth
+        s = m.sizes()
+        assert s["nb"] == 0
+    vptr, _ = quad.polygons()
+    assert set(np.diff(vptr)) == {4}
+
+
+def test_cell_average_rule(afar):
+    """The conical product rule with n points per direction integrates
+    polynomials of degree 2n - 2 exactly: two rules agree on a degree-8
+    polynomial over triangles and quads, and constants and linear functions
+    average to 1 and to the centroid."""
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    xi, eta, w = cases.triangle_rule(5)
+    assert abs(w.sum() - 0.5) < 1e-15
+    # int_T x^a y^b = a! b! / (a + b + 2)!
+    from math import factorial
+    for a, b in [(0, 0), (3, 4), (8, 0), (2, 6)]:
+        exact = factorial(a) * factorial(b) / factorial(a + b + 2)
+        assert abs((w * xi ** a * eta ** b).sum() - exact) < 1e-15
+    for split in (1.0, 0.0):
+        m = afar.Mesh.periodic(n=6, L=10.0, jitter=0.15, seed=1, split_p=split)
+        f = lambda x, y: np.stack([np.ones_like(x), x, y, (x - 3) ** 5 * (y - 2) ** 3 + x ** 8], 1)
+        a5 = cases.cell_average(m, f, n=5)
+        a7 = cases.cell_average(m, f, n=7)
+        assert np.allclose(a5[:, 3], a7[:, 3], rtol=1e-12, atol=1e-9)
+        xy, _ = m.cells()
+        c = np.empty((m.nc, 2))
+        c[m.permutation()] = xy[:m.nc]
+        assert np.abs(a5[:, 0] - 1).max() < 1e-13
+        assert np.abs(a5[:, 1:3] - c).max() < 1e-12
