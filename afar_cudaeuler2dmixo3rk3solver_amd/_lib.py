@@ -87,6 +87,10 @@ SIGNATURES = {
     "afar_mesh_generate_riemann": ([C.c_int32, C.c_double, C.c_uint64, C.c_double, P(_VP)], C.c_int),
     "afar_mesh_generate_periodic": ([C.c_int32, C.c_double, C.c_double, C.c_uint64, C.c_double, P(_VP)], C.c_int),
     "afar_mesh_destroy": ([_VP], None),
+    "afar_mesh_read_foam": ([C.c_char_p, C.c_int32, P(_VP)], C.c_int),
+    "afar_mesh_write_foam": ([_VP, C.c_char_p], C.c_int),
+    "afar_mesh_patch_info": ([_VP, C.c_int32, C.c_char_p, C.c_int32, _I32], C.c_int),
+    "afar_foam_read_fields": ([_VP, C.c_char_p, C.c_char_p, _D, _D, _I32], C.c_int),
     "afar_mesh_set_patch": ([_VP, C.c_int32, C.c_int32, _D], C.c_int),
     "afar_mesh_sizes": ([_VP, _I64], C.c_int),
     "afar_mesh_cells": ([_VP, _D, _D], C.c_int),
@@ -106,11 +110,12 @@ SIGNATURES = {
     "afar_solver_evolve_auto": ([_VP, C.c_int32], C.c_int),
     "afar_solver_last_dt": ([_VP, _D], C.c_int),
     "afar_solver_synchronize": ([_VP], C.c_int),
+    "afar_solver_query": ([_VP, _I32], C.c_int),
     "afar_solver_set_time": ([_VP, C.c_double, C.c_double], C.c_int),
-    "afar_solver_get_time": ([_VP, _D], C.c_int),
+    "afar_solver_get_time": ([_VP, _D, _D], C.c_int),
     "afar_solver_residual_norm": ([_VP, _D], C.c_int),
     "afar_batch_set_time": ([_VP, C.c_int32, C.c_double, C.c_double], C.c_int),
-    "afar_batch_get_time": ([_VP, _D], C.c_int),
+    "afar_batch_get_time": ([_VP, _D, _D], C.c_int),
     "afar_solver_write_field": ([_VP, _D, _D, _I8], C.c_int),
     "afar_solver_get_conserved": ([_VP, _D], C.c_int),
     "afar_solver_set_conserved": ([_VP, _D], C.c_int),

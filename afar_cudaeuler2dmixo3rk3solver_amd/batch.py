@@ -87,8 +87,13 @@ class BatchSolver:
 
     def device_time(self):
         t = np.zeros(len(self.meshes))
-        check(lib().afar_batch_get_time(self._h, dptr(t)))
+        check(lib().afar_batch_get_time(self._h, dptr(t), None))
         return t
+
+    def device_steps(self):
+        n = np.zeros(len(self.meshes))
+        check(lib().afar_batch_get_time(self._h, None, dptr(n)))
+        return n.astype(np.int64)
 
     def conserved(self, inst):
         U = np.zeros((self.meshes[inst].nc, 4))

@@ -11,6 +11,7 @@
 #include "afar_host.h"
 #include "afar_math.h"
 #include "kernels.h"
+#include "foam_io.h"
 #include "layout.h"
 #include "solver.h"
 
@@ -59,7 +60,7 @@ Schemes to_schemes(const afar_schemes* s) {
 }
 
 int tile_of(const afar_schemes* s) {
-    const int t = (s && s->tile) ? s->tile : 256;
+    const int t = (s && s->tile) ? s->tile : 64;
     need(t == 64 || t == 128 || t == 256, "tile must be 64, 128 or 256");
     return t;
 }
@@ -107,7 +108,7 @@ void afar_schemes_default(afar_schemes* s) {
     s->recalcBasis = 1;
     s->cOrder = 5;
     s->fOrder = 4;
-    s->tile = 256;
+    s->tile = 64;
 }
 
 // ---------------------------------------------------------------- meshes
@@ -158,6 +159,74 @@ int afar_mesh_from_polygons(int64_t npoints, const double* xy, int64_t ncells, c
     });
 }
 
+int afar_mesh_read_foam(const char* case_dir, int32_t renumber, afar_mesh** out) {
+    return guard([&] {
+        need(case_dir && out, "NULL argument");
+        PolyMesh pm;
+        try {
+            pm = read_polymesh_2d(case_dir, nullptr, nullptr);
+        } catch (const Error& e) {
+            throw Error(AFAR_ERR_IO, e.what());
+        }
+        finish_mesh(pm, renumber != 0, out);
+    });
+}
+
+int afar_mesh_write_foam(const afar_mesh* h, const char* case_dir) {
+    return guard([&] {
+        need(h && case_dir, "NULL argument");
+        try {
+            write_polymesh_2d(h->m, case_dir);
+        } catch (const Error& e) {
+            throw Error(AFAR_ERR_IO, e.what());
+        }
+    });
+}
+
+int afar_mesh_patch_info(const afar_mesh* h, int32_t patch, char* name, int32_t cap, int32_t* nfaces) {
+    return guard([&] {
+        need(h != nullptr, "NULL mesh");
+        need(patch >= 0 && patch < (int)h->m.patches.size(), "patch index out of range");
+        if (name && cap > 0) snprintf(name, cap, "%s", h->m.patches[patch].name.c_str());
+        if (nfaces) *nfaces = h->m.patch_size[patch];
+    });
+}
+
+int afar_foam_read_fields(const afar_mesh* h, const char* case_dir, const char* time_dir, double* W0,
+                          double* patch_values, int32_t* patch_fixed) {
+    return guard([&] {
+        need(h && case_dir && time_dir && W0, "NULL argument");
+        const Mesh& m = h->m;
+        std::vector<std::string> names;
+        for (const Patch& p : m.patches) names.push_back(p.name);
+        const std::string d = std::string(case_dir) + "/" + time_dir + "/";
+        FoamField fp, fU, fT;
+        try {
+            fp = read_field(d + "p", 1, m.nc, names);
+            fU = read_field(d + "U", 2, m.nc, names);
+            fT = read_field(d + "T", 1, m.nc, names);
+        } catch (const Error& e) {
+            throw Error(AFAR_ERR_IO, e.what());
+        }
+        need(fp.have_internal && fU.have_internal && fT.have_internal, "a field has no internalField");
+        for (int i = 0; i < m.nc; ++i) {
+            W0[4 * (size_t)i] = fp.internal[i];
+            W0[4 * (size_t)i + 1] = fU.internal[2 * (size_t)i];
+            W0[4 * (size_t)i + 2] = fU.internal[2 * (size_t)i + 1];
+            W0[4 * (size_t)i + 3] = fT.internal[i];
+        }
+        for (size_t p = 0; p < names.size(); ++p) {
+            if (patch_values) {
+                patch_values[4 * p] = fp.patch_value[p];
+                patch_values[4 * p + 1] = fU.patch_value[2 * p];
+                patch_values[4 * p + 2] = fU.patch_value[2 * p + 1];
+                patch_values[4 * p + 3] = fT.patch_value[p];
+            }
+            if (patch_fixed) patch_fixed[p] = fp.patch_fixed[p] && fU.patch_fixed[p] && fT.patch_fixed[p];
+        }
+    });
+}
+
 int afar_mesh_generate_corner(int32_t nx, int32_t ny, double lx, double ly, double jitter, uint64_t seed,
                               double corner_x, double angle_deg, const double* inflow, int32_t top_label,
                               afar_mesh** out) {
@@ -183,20 +252,14 @@ int afar_mesh_set_patch(afar_mesh* h, int32_t patch, int32_t label, const double
         need(h != nullptr, "NULL mesh");
         need(label >= 0 && label <= kMaxBCLabel, "boundary label out of range 0..6");
         Mesh& m = h->m;
-        // patch counts over non-empty patches, as the reference's patch loop does
-        int idx = -1;
-        for (size_t p = 0; p < m.patches.size(); ++p) {
-            if (m.patch_size[p] == 0) continue;
-            if (++idx == patch) {
-                m.patches[p].label = label;
-                if (value)
-                    for (int k = 0; k < 4; ++k) m.patches[p].value[k] = value[k];
-                for (int b = m.patch_start[p]; b < m.patch_start[p] + m.patch_size[p]; ++b)
-                    m.bf_label[b] = (uint8_t)label;
-                return;
-            }
-        }
-        throw Error(AFAR_ERR_INVALID, "patch index out of range");
+        // the patch index counts every patch, empty ones included, as the
+        // reference's patch<i> names do (basicCUDAsolver.cpp:157-172)
+        need(patch >= 0 && patch < (int)m.patches.size(), "patch index out of range");
+        m.patches[patch].label = label;
+        if (value)
+            for (int k = 0; k < 4; ++k) m.patches[patch].value[k] = value[k];
+        for (int b = m.patch_start[patch]; b < m.patch_start[patch] + m.patch_size[patch]; ++b)
+            m.bf_label[b] = (uint8_t)label;
     });
 }
 
@@ -213,6 +276,7 @@ int afar_mesh_sizes(const afar_mesh* h, int64_t* out) {
         out[5] = h->precomputed ? h->pc.st.maxC : 0;
         out[6] = h->maxLB;
         out[7] = h->maxCLB;
+        out[8] = (int64_t)h->m.patches.size();
     });
 }
 
@@ -584,6 +648,13 @@ int afar_batch_profile(afar_batch* b, int32_t reps, double* ms, int32_t* count) 
 
 void afar_batch_destroy(afar_batch* b) { delete b; }
 
+int afar_solver_query(afar_solver* h, int32_t* idle) {
+    return guard([&] {
+        need(h && idle, "NULL argument");
+        *idle = h->dev->idle() ? 1 : 0;
+    });
+}
+
 int afar_solver_set_time(afar_solver* h, double t, double t_end) {
     return guard([&] {
         need(h != nullptr, "NULL solver");
@@ -591,10 +662,10 @@ int afar_solver_set_time(afar_solver* h, double t, double t_end) {
     });
 }
 
-int afar_solver_get_time(afar_solver* h, double* t) {
+int afar_solver_get_time(afar_solver* h, double* t, double* steps) {
     return guard([&] {
-        need(h && t, "NULL argument");
-        h->dev->get_time(t);
+        need(h && (t || steps), "NULL argument");
+        h->dev->get_time(t, steps);
     });
 }
 
@@ -612,10 +683,10 @@ int afar_batch_set_time(afar_batch* b, int32_t inst, double t, double t_end) {
     });
 }
 
-int afar_batch_get_time(afar_batch* b, double* t) {
+int afar_batch_get_time(afar_batch* b, double* t, double* steps) {
     return guard([&] {
-        need(b && t, "NULL argument");
-        b->dev->get_time(t);
+        need(b && (t || steps), "NULL argument");
+        b->dev->get_time(t, steps);
     });
 }
 

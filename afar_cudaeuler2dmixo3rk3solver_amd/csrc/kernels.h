@@ -63,9 +63,9 @@ struct DevPtrs {
     int ninst;
     const int32_t* cinst;      // nc instance of each cell, or nullptr
     const int32_t* finst_off;  // ninst+1 first internal face of each instance
-    // Simulated time of each instance on the device path: tctl[2i] = t,
-    // tctl[2i+1] = end time.  The device time step is clipped to land on the
-    // end time; once there, steps leave the state unchanged (time step 0).
+    // Device clock of each instance: tctl[4i] = t, tctl[4i+1] = end time,
+    // tctl[4i+2] = steps taken.  The device time step is clipped to land on
+    // the end time; once there, steps leave the state unchanged (time step 0).
     double* tctl;
     double* res_part;          // partial sums of the residual norm (one per update block)
 };

@@ -766,12 +766,14 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(double* dt, const double* pa
         for (int w = 1; w < 16; ++w) m = wmin[w] < m ? wmin[w] : m;
         // clip to the end time as the reference's caller does with
         // Foam::Time (dt = min(dt, t_end - t)); 0 once the end is reached
-        const double t = tctl[2 * inst], te = tctl[2 * inst + 1];
+        double* tc = tctl + 4 * inst;
+        const double t = tc[0], te = tc[1];
         double d = 0.0;
         if (t < te) {
             const double rem = te - t;
             d = m < rem ? m : rem;
-            tctl[2 * inst] = t + d;
+            tc[0] = t + d;
+            tc[2] += 1.0;  // steps taken
         }
         dt[inst] = d;
     }

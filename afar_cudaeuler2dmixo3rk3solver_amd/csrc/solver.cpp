@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <cstdio>
+#include <mutex>
 
 #include "kernels.h"
 
@@ -68,7 +69,14 @@ class SolverImpl final : public DeviceSolver {
         if (lds_recon_ > lds_cap)
             throw Error(3, "Too large local block stencil size: reconstruct needs " + std::to_string(lds_recon_) +
                                " bytes of LDS, the device allows " + std::to_string(lds_cap));
-        check_hip(set_lds_limits<R>(lds_recon_ > 65536 ? lds_recon_ : 65536), "hipFuncSetAttribute");
+        // The reconstruction kernels' dynamic-LDS limit is raised to the
+        // device's per-block maximum once per process: the attribute is
+        // global per kernel, so it is never changed while other solvers (on
+        // other host threads) may be launching those kernels.
+        static std::once_flag lds_once;
+        static hipError_t lds_err = hipSuccess;
+        std::call_once(lds_once, [&] { lds_err = set_lds_limits<R>(lds_cap); });
+        check_hip(lds_err, "hipFuncSetAttribute");
         upload();
     }
     ~SolverImpl() override {
@@ -104,6 +112,13 @@ class SolverImpl final : public DeviceSolver {
     }
 
     void synchronize() override { check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+    bool idle() override {
+        const hipError_t e = hipStreamQuery(stream_);
+        if (e == hipErrorNotReady) return false;
+        check_hip(e, "hipStreamQuery");
+        return true;
+    }
 
     int ninst() const override { return bi_.ninst(); }
 
@@ -171,26 +186,30 @@ class SolverImpl final : public DeviceSolver {
     void set_time(int inst, double t, double t_end) override {
         if (inst < -1 || inst >= ninst()) throw Error(1, "instance out of range");
         if (!(t_end >= t)) throw Error(1, "end time before the current time");
-        std::vector<double> tc(2 * (size_t)ninst());
+        std::vector<double> tc(4 * (size_t)ninst());
         check_hip(hipMemcpyAsync(tc.data(), P_.tctl, sizeof(double) * tc.size(), hipMemcpyDeviceToHost, stream_),
                   "clock copy");
         synchronize();
         for (int k = 0; k < ninst(); ++k)
             if (inst < 0 || inst == k) {
-                tc[2 * k] = t;
-                tc[2 * k + 1] = t_end;
+                tc[4 * k] = t;
+                tc[4 * k + 1] = t_end;
+                tc[4 * k + 2] = 0.0;
             }
         check_hip(hipMemcpyAsync(P_.tctl, tc.data(), sizeof(double) * tc.size(), hipMemcpyHostToDevice, stream_),
                   "clock copy");
         synchronize();
     }
 
-    void get_time(double* t) override {
-        std::vector<double> tc(2 * (size_t)ninst());
+    void get_time(double* t, double* steps) override {
+        std::vector<double> tc(4 * (size_t)ninst());
         check_hip(hipMemcpyAsync(tc.data(), P_.tctl, sizeof(double) * tc.size(), hipMemcpyDeviceToHost, stream_),
                   "clock copy");
         synchronize();
-        for (int k = 0; k < ninst(); ++k) t[k] = tc[2 * k];
+        for (int k = 0; k < ninst(); ++k) {
+            if (t) t[k] = tc[4 * k];
+            if (steps) steps[k] = tc[4 * k + 2];
+        }
     }
 
     double residual_norm() override {
@@ -459,11 +478,8 @@ class SolverImpl final : public DeviceSolver {
         }
         P_.finst_off = put(m_finst, bi_.face_off.data(), ni + 1);
         {
-            std::vector<double> tc(2 * (size_t)ni);
-            for (int k = 0; k < ni; ++k) {
-                tc[2 * k] = 0.0;
-                tc[2 * k + 1] = HUGE_VAL;
-            }
+            std::vector<double> tc(4 * (size_t)ni, 0.0);
+            for (int k = 0; k < ni; ++k) tc[4 * k + 1] = HUGE_VAL;
             P_.tctl = put(m_tctl, tc.data(), tc.size());
         }
         m_res.alloc(sizeof(double) * (residual_blocks(L_.nc) + 1));

@@ -15,7 +15,7 @@ struct SolverConfig {
     double R = 0.7142857, Cv = 1.7857142857, CFL = 0.5;
     int precision = 0;  // 0 FP64, 1 FP32-mixed
     int exactRK = 0;
-    int tile = 256;
+    int tile = 64;
     int boundaryFirstOrder = 0;
     int reconLanes = 0;  // reconstruction threads per cell: 0 automatic, 1, or 4
 };
@@ -42,6 +42,7 @@ class DeviceSolver {
     virtual void evolve(double dt) = 0;                 // one SSP-RK3 step with a host time step
     virtual void evolve_auto(int nsteps) = 0;           // CFL + step on the device, graph replay
     virtual void synchronize() = 0;
+    virtual bool idle() = 0;  // all enqueued work done (does not wait)
     virtual void copy_out(double* FD, int8_t* shock) = 0;  // FD 4*(nc+nb); shock 2*(nc+nb) interleaved
     virtual void get_conserved(double* U) = 0;          // 4*nc
     virtual void set_conserved(const double* U) = 0;    // also recomputes W and ghosts
@@ -57,9 +58,10 @@ class DeviceSolver {
     // set_conserved); the device clock is not changed.
     virtual void reset_state() = 0;
     // Device clock of the device-time-step path: instance inst (-1: all) is
-    // at time t and stops at t_end (steps past it leave the state alone).
+    // at time t and stops at t_end (steps past it leave the state alone);
+    // its count of steps taken restarts at 0.
     virtual void set_time(int inst, double t, double t_end) = 0;
-    virtual void get_time(double* t) = 0;  // ninst values
+    virtual void get_time(double* t, double* steps) = 0;  // ninst values each; either may be NULL
     // Root mean square over cells of the density residual per unit volume,
     // |sum_f F_rho| / V, of the current state (runs the stage kernels, so the
     // switch flags become those of the current state).

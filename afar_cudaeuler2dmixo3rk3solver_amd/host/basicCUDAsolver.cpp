@@ -65,9 +65,34 @@ std::string runTime::timeName() const {
 
 // ------------------------------------------------------------------ caseMesh
 
+static bool file_exists(const std::string& path) {
+    FILE* f = fopen(path.c_str(), "r");
+    if (f) fclose(f);
+    return f != nullptr;
+}
+
 caseMesh::caseMesh(const std::string& caseDir) {
     schemes_ = dictionary::fromFile(caseDir + "/system/fvSchemes");
-    const dictionary md = dictionary::fromFile(caseDir + "/system/afarMeshDict");
+    const bool haveMd = file_exists(caseDir + "/system/afarMeshDict");
+    const bool havePoly = file_exists(caseDir + "/constant/polyMesh/faces");
+    const dictionary md = haveMd ? dictionary::fromFile(caseDir + "/system/afarMeshDict") : dictionary();
+    if (havePoly && (!haveMd || md.lookupWord("mesh.type") == "polyMesh")) {
+        // an OpenFOAM case: constant/polyMesh and the 0/ fields, as the reference reads them
+        // (basicCUDAsolver.cpp:9-54,122-258); renumbering is left to the case (renumberMesh)
+        // unless afarMeshDict asks for it
+        const bool renumber = haveMd && md.lookupOrDefault("mesh.renumber", false);
+        check(afar_mesh_read_foam(caseDir.c_str(), renumber ? 1 : 0, &mesh_), "read polyMesh");
+        int64_t s[9];
+        check(afar_mesh_sizes(mesh_, s), "afar_mesh_sizes");
+        nc_ = s[0];
+        W0_.assign(4 * (size_t)nc_, 0.0);
+        patchValues_.assign(4 * (size_t)s[8], 0.0);
+        std::vector<int32_t> fixed(s[8], 0);
+        check(afar_foam_read_fields(mesh_, caseDir.c_str(), "0", W0_.data(), patchValues_.data(), fixed.data()),
+              "read 0/ fields");
+        patchFixed_.assign(fixed.begin(), fixed.end());
+        return;
+    }
     const dictionary& m = md.subDict("mesh");
     const std::string type = m.lookupWord("type");
     const long seed = m.lookupOrDefault("seed", 0L);
@@ -91,7 +116,7 @@ caseMesh::caseMesh(const std::string& caseDir) {
     } else {
         throw solverError(AFAR_ERR_INVALID, "unknown mesh type " + type);
     }
-    int64_t s[8];
+    int64_t s[9];
     check(afar_mesh_sizes(mesh_, s), "afar_mesh_sizes");
     nc_ = s[0];
     std::vector<double> c(2 * (size_t)(s[0] + s[2]));
@@ -169,7 +194,7 @@ afar_schemes readSchemes(const dictionary& fv, const dictionary& control) {
     const std::string prec = fv.found("CUDA.precision") ? fv.lookupWord("CUDA.precision") : "fp64";
     s.precision = prec == "mixed" ? 1 : 0;
     s.exactRK = fv.lookupOrDefault("CUDA.exactRK", false);
-    s.tile = (int32_t)fv.lookupOrDefault("CUDA.tile", 256L);
+    s.tile = (int32_t)fv.lookupOrDefault("CUDA.tile", 64L);
     s.boundaryFirstOrder = fv.lookupOrDefault("CUDA.boundaryFirstOrder", false);
     return s;
 }
@@ -179,16 +204,28 @@ afar_schemes readSchemes(const dictionary& fv, const dictionary& control) {
 basicCUDAsolver::basicCUDAsolver(const runTime& rt, const caseMesh& mesh) {
     sc_ = readSchemes(mesh.schemes(), rt.controlDict());
     afar_mesh* m = mesh.mesh();
-    // boundary labels CUDA.boundary.patch<i> (basicCUDAsolver.cpp:170-172), one per non-empty patch
-    int64_t s[8];
+    // boundary labels CUDA.boundary.patch<i> for every non-empty patch, i counting all
+    // patches (basicCUDAsolver.cpp:157-172); label 6 takes CUDA.boundary.value<i>, else the
+    // patch's fixedValue values of the 0/ fields
+    int64_t s[9];
     check(afar_mesh_sizes(m, s), "afar_mesh_sizes");
-    for (int p = 0; p < s[3]; ++p) {
+    for (int p = 0; p < s[8]; ++p) {
+        int32_t nfaces = 0;
+        check(afar_mesh_patch_info(m, p, nullptr, 0, &nfaces), "afar_mesh_patch_info");
+        if (nfaces == 0) continue;
         const std::string key = "CUDA.boundary.patch" + std::to_string(p);
         const long label = mesh.schemes().lookupLabel(key);
         std::vector<double> val(4, 0.0);
         const std::string vkey = "CUDA.boundary.value" + std::to_string(p);
-        if (mesh.schemes().found(vkey)) val = vec_of(mesh.schemes(), vkey, 4);
-        check(afar_mesh_set_patch(m, p, (int32_t)label, label == 6 && mesh.schemes().found(vkey) ? val.data() : nullptr),
+        bool haveVal = false;
+        if (mesh.schemes().found(vkey)) {
+            val = vec_of(mesh.schemes(), vkey, 4);
+            haveVal = true;
+        } else if (mesh.patchFixed(p)) {
+            val = mesh.patchValue(p);
+            haveVal = true;
+        }
+        check(afar_mesh_set_patch(m, p, (int32_t)label, label == 6 && haveVal ? val.data() : nullptr),
               "afar_mesh_set_patch");
     }
     const std::string basisDir = rt.path() + "/system";

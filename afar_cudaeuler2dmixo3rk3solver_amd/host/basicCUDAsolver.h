@@ -51,8 +51,9 @@ class runTime {
 };
 
 // The mesh, schemes and initial fields the reference takes from
-// Foam::fvMesh and the 0/ directory.  The mesh is described by
-// system/afarMeshDict (generator and initial condition).
+// Foam::fvMesh and the 0/ directory: an OpenFOAM case (constant/polyMesh and
+// 0/{p,U,T}, ASCII, one cell thick), or system/afarMeshDict (a generator and
+// an initial condition).
 class caseMesh {
    public:
     explicit caseMesh(const std::string& caseDir);
@@ -65,10 +66,17 @@ class caseMesh {
     int64_t nCells() const { return nc_; }
     const std::vector<double>& cellCentres() const { return centres_; }  // caller order
 
+    // fixedValue values (p, Ux, Uy, T) of patch p in the 0/ fields of an OpenFOAM case
+    bool patchFixed(int p) const { return p < (int)patchFixed_.size() && patchFixed_[p]; }
+    std::vector<double> patchValue(int p) const {
+        return std::vector<double>(patchValues_.begin() + 4 * p, patchValues_.begin() + 4 * p + 4);
+    }
+
    private:
     afar_mesh* mesh_ = nullptr;
     dictionary schemes_;
-    std::vector<double> W0_, centres_;
+    std::vector<double> W0_, centres_, patchValues_;
+    std::vector<char> patchFixed_;
     int64_t nc_ = 0;
 };
 

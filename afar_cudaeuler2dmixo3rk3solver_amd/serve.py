@@ -20,6 +20,7 @@ cell-steps/s and the L1 density error of a sample of requests against the
 all-FP64 solve.
 """
 import argparse
+import faulthandler
 import json
 import math
 import queue
@@ -98,64 +99,78 @@ def footprint_bytes(mesh, precision):
 
 
 class Server:
-    """Continuous-batching request loop (see the module docstring)."""
+    """Continuous-batching request loop (see the module docstring).
+
+    Host threads only prepare requests (mesh generation and precompute, no
+    GPU calls).  One scheduler thread makes every GPU call: it admits
+    prepared requests while their footprint fits in the HBM budget and a slot
+    is free, enqueues chunks of graph-launched steps on each active solver's
+    stream without waiting, and polls the streams; a request whose stream is
+    idle either gets its next chunk, or is finished (result copied back,
+    solver destroyed, memory and slot released)."""
 
     def __init__(self, precision="mixed", hbm_budget=64 << 30, max_active=8, chunk=50, prep_workers=2,
-                 log=None):
+                 log=None, poll_s=2e-4):
         self.precision = precision
         self.budget = hbm_budget
         self.max_active = max_active
         self.chunk = chunk
         self.prep_workers = prep_workers
+        self.poll_s = poll_s
         self.log = log or (lambda m: None)
         self.sc = schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=precision)
         self.sc_riemann = schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=precision, boundaryFirstOrder=True)
-        self._cv = threading.Condition()
-        self._used = 0
-        self._active = 0
         self.peak_active = 0
 
-    def _admit(self, need):
-        with self._cv:
-            while self._active >= self.max_active or (self._used + need > self.budget and self._active > 0):
-                self._cv.wait()
-            self._used += need
-            self._active += 1
-            self.peak_active = max(self.peak_active, self._active)
+    def schemes_of(self, req):
+        return self.sc if req.kind == "corner" else self.sc_riemann
 
-    def _release(self, need):
-        with self._cv:
-            self._used -= need
-            self._active -= 1
-            self._cv.notify_all()
+    # -- per-request steps, all on the scheduler thread
+    def _start(self, req, mesh, W0, t0):
+        g = BasicCUDASolver(mesh, self.schemes_of(req), W0)
+        req.times["start"] = time.perf_counter() - t0
+        req.cells = mesh.nc
+        if req.kind == "riemann":
+            g.set_time(0.0, req.t_end)
+        self._enqueue(req, g)
+        return g
+
+    def _enqueue(self, req, g):
+        k = self.chunk if req.kind == "riemann" else min(self.chunk, req.max_steps - req.steps)
+        g.evolve_auto(k)
+        req.steps += k
+
+    def _advance(self, req, g):
+        """Called when the solver's stream is idle: True when the request is done."""
+        if req.kind == "riemann":
+            if g.device_time() >= req.t_end:
+                req.steps = g.device_steps()  # steps that moved the clock
+                return True
+        else:
+            req.residual = g.residual_norm()
+            if req.residual < req.res_tol or req.steps >= req.max_steps:
+                return True
+        self._enqueue(req, g)
+        return False
+
+    def _finish(self, req, g, t0):
+        req.result = g.conserved()
+        g.close()
+        req.times["finish"] = time.perf_counter() - t0
 
     def solve(self, req, mesh, W0, t0):
-        """Runs one admitted request on the GPU (own solver, own stream)."""
-        sc = self.sc if req.kind == "corner" else self.sc_riemann
-        g = BasicCUDASolver(mesh, sc, W0)
-        req.times["start"] = time.perf_counter() - t0
-        if req.kind == "riemann":
-            req.steps = g.run_until(req.t_end, chunk=self.chunk)
-        else:
-            n = 0
-            while n < req.max_steps:
-                k = min(self.chunk, req.max_steps - n)
-                g.evolve_auto(k)
-                n += k
-                req.residual = g.residual_norm()
-                if req.residual < req.res_tol:
-                    break
-            req.steps = n
-        req.result = g.conserved()
-        req.cells = mesh.nc
-        g.close()
+        """One request alone, with the same chunking as in the loop."""
+        g = self._start(req, mesh, W0, t0)
+        while True:
+            g.synchronize()
+            if self._advance(req, g):
+                break
+        self._finish(req, g, t0)
 
     def run(self, requests, realtime=True):
         t0 = time.perf_counter()
         prep_q = queue.Queue()
-        threads = []
-        results_lock = threading.Lock()
-        done = []
+        ready_q = queue.Queue()
 
         def arrivals():
             for r in requests:
@@ -167,41 +182,73 @@ class Server:
             for _ in range(self.prep_workers):
                 prep_q.put(None)
 
-        def runner(r, mesh, W0, need):
-            try:
-                self.solve(r, mesh, W0, t0)
-            finally:
-                self._release(need)
-            r.times["finish"] = time.perf_counter() - t0
-            with results_lock:
-                done.append(r)
-            self.log(json.dumps(summary_row(r)))
-
         def preparer():
             while True:
                 r = prep_q.get()
                 if r is None:
+                    ready_q.put(None)
                     return
                 r.times["prep_start"] = time.perf_counter() - t0
-                mesh, W0 = build_case(r, self.sc if r.kind == "corner" else self.sc_riemann)
+                mesh, W0 = build_case(r, self.schemes_of(r))
                 r.times["prepared"] = time.perf_counter() - t0
-                need = footprint_bytes(mesh, self.precision)
-                self._admit(need)
-                r.times["admitted"] = time.perf_counter() - t0
-                th = threading.Thread(target=runner, args=(r, mesh, W0, need))
-                th.start()
-                threads.append(th)
+                ready_q.put((r, mesh, W0, footprint_bytes(mesh, self.precision)))
 
-        feeder = threading.Thread(target=arrivals)
+        feeder = threading.Thread(target=arrivals, daemon=True)
+        preps = [threading.Thread(target=preparer, daemon=True) for _ in range(self.prep_workers)]
         feeder.start()
-        preps = [threading.Thread(target=preparer) for _ in range(self.prep_workers)]
         for p in preps:
             p.start()
+
+        waiting = []   # prepared, not admitted (arrival order)
+        active = []    # (request, solver, footprint)
+        done = []
+        used = 0
+        producers = self.prep_workers
+        while producers > 0 or waiting or active:
+            progressed = False
+            while True:  # collect prepared requests
+                try:
+                    item = ready_q.get_nowait()
+                except queue.Empty:
+                    break
+                if item is None:
+                    producers -= 1
+                else:
+                    waiting.append(item)
+            while waiting and len(active) < self.max_active and (used + waiting[0][3] <= self.budget or not active):
+                r, mesh, W0, need = waiting.pop(0)
+                r.times["admitted"] = time.perf_counter() - t0
+                g = self._start(r, mesh, W0, t0)
+                active.append((r, g, need))
+                used += need
+                self.peak_active = max(self.peak_active, len(active))
+                progressed = True
+            still = []
+            for r, g, need in active:
+                if g.idle() and self._advance(r, g):
+                    self._finish(r, g, t0)
+                    used -= need
+                    done.append(r)
+                    self.log(json.dumps(summary_row(r)))
+                    progressed = True
+                else:
+                    still.append((r, g, need))
+            active = still
+            if not progressed:
+                if producers > 0 and not active and not waiting:
+                    try:
+                        item = ready_q.get(timeout=0.05)
+                        if item is None:
+                            producers -= 1
+                        else:
+                            waiting.append(item)
+                    except queue.Empty:
+                        pass
+                else:
+                    time.sleep(self.poll_s)
         feeder.join()
         for p in preps:
             p.join()
-        for th in list(threads):
-            th.join()
         return sorted(done, key=lambda r: r.rid), time.perf_counter() - t0
 
 
@@ -214,18 +261,18 @@ def summary_row(r):
                 residual=None if math.isnan(r.residual) else r.residual)
 
 
-def fp64_reference(req):
+def fp64_reference(req, chunk=50):
     """The request solved alone on the all-FP64 path (the FP64 GPU path is
     pinned to the CPU oracle elsewhere), with the same stepping."""
-    srv = Server(precision="fp64", max_active=1)
-    sc = srv.sc if req.kind == "corner" else srv.sc_riemann
-    mesh, W0 = build_case(req, sc)
+    srv = Server(precision="fp64", max_active=1, chunk=chunk)
+    mesh, W0 = build_case(req, srv.schemes_of(req))
     r = Request(req.rid, 0.0, req.kind, req.nc_target, req.seed, req.t_end, req.res_tol, req.max_steps)
     srv.solve(r, mesh, W0, time.perf_counter())
     return r
 
 
 def main(argv=None):
+    faulthandler.enable()
     ap = argparse.ArgumentParser()
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=120.0)
@@ -249,7 +296,7 @@ def main(argv=None):
     sample = [r for r in done if rng.uniform() < args.sample]
     errs = []
     for r in sample:
-        ref = fp64_reference(r)
+        ref = fp64_reference(r, args.chunk)
         d = np.abs(r.result[:, 0] - ref.result[:, 0]).sum() / np.abs(ref.result[:, 0]).sum()
         errs.append(float(d))
     print(json.dumps(dict(

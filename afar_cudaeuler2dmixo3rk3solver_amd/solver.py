@@ -18,7 +18,7 @@ STENCIL_TYPES = {"faceNeighbour": 0, "edgeNeighbour": 1}
 
 
 def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilType="edgeNeighbour",
-            recalcBasis=True, cOrder=5, fOrder=4, precision="fp64", exactRK=False, tile=256,
+            recalcBasis=True, cOrder=5, fOrder=4, precision="fp64", exactRK=False, tile=64,
             boundaryFirstOrder=False, reconLanes=0):
     """afar_schemes with the reference's key names and defaults (the last
     five are this library's extensions)."""
@@ -76,6 +76,47 @@ class Mesh:
         return cls(h)
 
     @classmethod
+    def read_foam(cls, case_dir, renumber=False):
+        """The 2D mesh of an OpenFOAM case (constant/polyMesh, ASCII, one cell
+        thick); patches in polyMesh order, label 4 until set_patch."""
+        h = C.c_void_p()
+        check(lib().afar_mesh_read_foam(str(case_dir).encode(), int(bool(renumber)), C.byref(h)))
+        return cls(h)
+
+    def write_foam(self, case_dir):
+        """Writes case_dir/constant/polyMesh (created if missing)."""
+        import os
+        os.makedirs(os.path.join(str(case_dir), "constant", "polyMesh"), exist_ok=True)
+        check(lib().afar_mesh_write_foam(self._h, str(case_dir).encode()))
+
+    def patches(self):
+        """[(name, boundary faces)] of every patch, empty ones included."""
+        out = []
+        for p in range(self.sizes()["npatches_all"]):
+            buf = C.create_string_buffer(256)
+            n = C.c_int32()
+            check(lib().afar_mesh_patch_info(self._h, p, buf, 256, C.byref(n)))
+            out.append((buf.value.decode(), n.value))
+        return out
+
+    def set_patch(self, patch, label, value=None):
+        """Boundary label of patch `patch` (CUDA.boundary.patch<i>); value
+        (p, Ux, Uy, T) for label 6."""
+        v = None if value is None else np.ascontiguousarray(value, dtype=np.float64)
+        check(lib().afar_mesh_set_patch(self._h, patch, label, None if v is None else dptr(v)))
+
+    def read_foam_fields(self, case_dir, time_dir="0"):
+        """(W0 per cell in file order, per-patch fixedValue values, fixed flags)
+        from the case's p, U and T."""
+        n = self.sizes()["npatches_all"]
+        W0 = np.zeros((self.nc, 4))
+        pv = np.zeros((n, 4))
+        pf = np.zeros(n, dtype=np.int32)
+        check(lib().afar_foam_read_fields(self._h, str(case_dir).encode(), str(time_dir).encode(), dptr(W0),
+                                          dptr(pv), ptr(pf, C.c_int32)))
+        return W0, pv, pf.astype(bool)
+
+    @classmethod
     def from_polygons(cls, points, cell_ptr, cell_pts, bedges, bedge_patch, patch_labels, patch_values=None,
                       cell_vshift=None, period=(0.0, 0.0), depth=1.0, renumber=True):
         pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
@@ -95,9 +136,9 @@ class Mesh:
 
     # -- queries
     def sizes(self):
-        o = np.zeros(8, dtype=np.int64)
+        o = np.zeros(9, dtype=np.int64)
         check(lib().afar_mesh_sizes(self._h, ptr(o, C.c_int64)))
-        keys = ["nc", "nf", "nb", "npatches", "maxS", "maxC", "maxLB", "maxCLB"]
+        keys = ["nc", "nf", "nb", "npatches", "maxS", "maxC", "maxLB", "maxCLB", "npatches_all"]
         return dict(zip(keys, (int(v) for v in o)))
 
     @property
@@ -258,13 +299,25 @@ class BasicCUDASolver:
     def synchronize(self):
         check(lib().afar_solver_synchronize(self._h))
 
+    def idle(self):
+        """True when every enqueued step has finished (does not wait)."""
+        v = C.c_int32()
+        check(lib().afar_solver_query(self._h, C.byref(v)))
+        return bool(v.value)
+
     def set_time(self, t, t_end=float("inf")):
         check(lib().afar_solver_set_time(self._h, t, t_end))
 
     def device_time(self):
         t = C.c_double()
-        check(lib().afar_solver_get_time(self._h, C.byref(t)))
+        check(lib().afar_solver_get_time(self._h, C.byref(t), None))
         return t.value
+
+    def device_steps(self):
+        """Steps taken on the device clock since the last set_time."""
+        n = C.c_double()
+        check(lib().afar_solver_get_time(self._h, None, C.byref(n)))
+        return int(n.value)
 
     def run_until(self, t_end, chunk=64, max_steps=None):
         """Device-clocked steps from the current device time to t_end (the
@@ -331,7 +384,7 @@ class BasicCUDASolver:
 class DeviceFields:
     """The reference's host<->device functions (GPUevolve.h) over afar_device."""
 
-    def __init__(self, a, precision=0, tile=256):
+    def __init__(self, a, precision=0, tile=64):
         s = a["sizes"]
         self.a = a
         self.s = s

@@ -92,7 +92,9 @@ def main():
     ap.add_argument("--case", default="riemann", choices=["riemann", "corner", "batch"],
                     help="batch: 256 corner flows (5k-200k cells, seed 2), FP32-mixed, autotuned per size bucket")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--tile", type=int, default=256)
+    ap.add_argument("--tile", type=int, default=64)
+    ap.add_argument("--recon-lanes", type=int, default=0, choices=[0, 1, 4],
+                    help="reconstruction threads per cell (0: 4 when every cell has at most four faces)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--faithful-boundary", action="store_true",
                     help="riemann case without the boundaryFirstOrder extension (loses positivity)")
@@ -132,7 +134,8 @@ def main():
     # unmodified reference algorithm within a few steps; the bench runs it with
     # the boundaryFirstOrder extension so the timed steps act on a physical state.
     bfo = args.case == "riemann" and not args.faithful_boundary
-    sc = A.schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=args.precision, tile=args.tile, boundaryFirstOrder=bfo)
+    sc = A.schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=args.precision, tile=args.tile, boundaryFirstOrder=bfo,
+                   reconLanes=args.recon_lanes)
     mesh.precompute(sc)
     t_pre = time.perf_counter() - t0
     print("bench: %s, nc=%d, precompute %.1f s" % (args.case, mesh.nc, t_pre), file=sys.stderr, flush=True)
@@ -178,7 +181,8 @@ def main():
         "config": {"workload": workload, "case": args.case, "cells": mesh.nc, "internal_faces": st["nf"],
                    "boundary_faces": st["nb"], "mean_stencil": round(st["meanS"], 3),
                    "mean_compact": round(st["meanC"], 3), "precision": args.precision, "cfl": 0.5,
-                   "tile": args.tile, "device": dev_name.strip(), "precompute_s": round(t_pre, 1),
+                   "tile": args.tile, "recon_lanes": args.recon_lanes or "auto", "device": dev_name.strip(),
+                   "precompute_s": round(t_pre, 1),
                    "boundary_first_order": bfo, "state_positive_after_timed_steps": finite},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -57,7 +57,7 @@ typedef struct afar_schemes {
     int32_t fOrder;
     int32_t precision;   /* 0 all-FP64, 1 FP32 reconstruction/limiter/flux with FP64 state */
     int32_t exactRK;     /* 0 the reference's RK3 literals (GPUevolve.cu:223-225), 1 exact 1/3, 2/3 */
-    int32_t tile;        /* cells per LDS tile / workgroup: 64, 128 or 256 (0 = 256) */
+    int32_t tile;        /* cells per LDS tile: 64, 128 or 256 (0 = 64) */
     int32_t boundaryFirstOrder; /* extension, 0 = reference behaviour: cells with a boundary face
                                    use first-order face states (the reference does this only for
                                    cells with two boundary faces, cudaLimiter.h:419-429) */
@@ -99,14 +99,34 @@ int afar_mesh_generate_riemann(int32_t n, double jitter, uint64_t seed, double s
  * into two triangles with probability split_p (1: 2n^2 triangles). */
 int afar_mesh_generate_periodic(int32_t n, double L, double jitter, uint64_t seed, double split_p, afar_mesh** out);
 void afar_mesh_destroy(afar_mesh* m);
-/* Sets the boundary label of the patch-th non-empty patch, as the reference
- * reads CUDA.boundary.patch<i> (basicCUDAsolver.cpp:170-172); value (p, Ux,
+/* OpenFOAM case I/O (ASCII), one-cell-thick 2D cases as the reference runs
+ * (basicCUDAsolver.cpp:9-54 reads them through Foam::fvMesh):
+ * afar_mesh_read_foam reads case_dir/constant/polyMesh (points, faces, owner,
+ * neighbour, boundary); the 2D cells are the cells' faces on the lower z
+ * plane, the depth is the z extent, patches keep the polyMesh order (empty
+ * ones included) and start with label 4 (set them with afar_mesh_set_patch
+ * from CUDA.boundary.patch<i>).  renumber 1 applies RCM (as renumberMesh),
+ * 0 keeps the file's cell order.  afar_mesh_write_foam writes a mesh as a
+ * one-cell-thick polyMesh into case_dir/constant/polyMesh (which must exist),
+ * with an extra empty patch frontAndBack.  afar_foam_read_fields reads p, U, T
+ * of case_dir/time_dir: W0 (4 per cell, file order = caller order) and per
+ * patch the uniform fixedValue values (4 per patch, patch_fixed 1 when p, U
+ * and T are all fixedValue); the last two may be NULL. */
+int afar_mesh_read_foam(const char* case_dir, int32_t renumber, afar_mesh** out);
+int afar_mesh_write_foam(const afar_mesh* m, const char* case_dir);
+/* Name (truncated to cap bytes) and boundary faces of patch `patch`; either output may be NULL. */
+int afar_mesh_patch_info(const afar_mesh* m, int32_t patch, char* name, int32_t cap, int32_t* nfaces);
+int afar_foam_read_fields(const afar_mesh* m, const char* case_dir, const char* time_dir, double* W0,
+                          double* patch_values, int32_t* patch_fixed);
+/* Sets the boundary label of patch `patch` (counting every patch, empty ones
+ * included, as the reference's CUDA.boundary.patch<i> names do,
+ * basicCUDAsolver.cpp:157-172); value (p, Ux,
  * Uy, T) is used by label 6 and may be NULL to keep the current one. */
 int afar_mesh_set_patch(afar_mesh* m, int32_t patch, int32_t label, const double* value);
-/* out[0..7] = nc, nf, nb, npatches, maxS, maxC, maxLocalBlockStencilSize,
- * maxCompactLocalBlockStencilSize (the last four are 0 before precompute;
- * the block sizes are those of the reference's 128-cell tables,
- * basicCUDAsolver.cpp:697-850). */
+/* out[0..8] = nc, nf, nb, non-empty patches, maxS, maxC,
+ * maxLocalBlockStencilSize, maxCompactLocalBlockStencilSize (these four are 0
+ * before precompute; the block sizes are those of the reference's 128-cell
+ * tables, basicCUDAsolver.cpp:697-850), all patches (empty ones included). */
 int afar_mesh_sizes(const afar_mesh* m, int64_t* out);
 /* Cell and pseudo-cell centres 2*(nc+nb), volumes nc (mesh order). */
 int afar_mesh_cells(const afar_mesh* m, double* centres, double* volumes);
@@ -154,13 +174,16 @@ int afar_solver_last_dt(afar_solver* h, double* dt);
 /* Device clock of evolve_auto: the solver is at time t and its device time
  * steps are clipped to end exactly at t_end (dt = min(dt, t_end - t), as the
  * reference application's Foam::Time does); steps past t_end leave the state
- * unchanged.  Initially t = 0, t_end = +inf. */
+ * unchanged.  Initially t = 0, t_end = +inf.  get_time gives the time and the
+ * number of steps taken since the last set_time (either pointer may be NULL). */
 int afar_solver_set_time(afar_solver* h, double t, double t_end);
-int afar_solver_get_time(afar_solver* h, double* t);
+int afar_solver_get_time(afar_solver* h, double* t, double* steps);
 /* sqrt(mean over cells of (sum_f F_rho / V)^2) for the current state (runs
  * the stage kernels; the switch flags become those of the current state). */
 int afar_solver_residual_norm(afar_solver* h, double* norm);
 int afar_solver_synchronize(afar_solver* h);
+/* *idle = 1 when all work enqueued on the solver's stream has finished, else 0; does not wait. */
+int afar_solver_query(afar_solver* h, int32_t* idle);
 /* basicCUDAsolver::writeField (basicCUDAsolver.cpp:874-913): W 4*nc and
  * shock nc (final reconstruction type, shockIndicator[2c+1]) in the caller's
  * order, ghost 4*nb in boundary-face order.  NULL pointers are skipped. */
@@ -209,9 +232,9 @@ int afar_batch_sizes(afar_batch* b, int64_t* out);
 int afar_batch_cell_offsets(afar_batch* b, int32_t* off);
 /* dt[n]: the time step each mesh took in its last step. */
 int afar_batch_last_dt(afar_batch* b, double* dt);
-/* Device clocks as afar_solver_set_time, for mesh inst (-1: all); t[n]. */
+/* Device clocks as afar_solver_set_time, for mesh inst (-1: all); t[n], steps[n]. */
 int afar_batch_set_time(afar_batch* b, int32_t inst, double t, double t_end);
-int afar_batch_get_time(afar_batch* b, double* t);
+int afar_batch_get_time(afar_batch* b, double* t, double* steps);
 /* Conserved state (4 per cell) of mesh inst, its caller order. */
 int afar_batch_get_conserved(afar_batch* b, int32_t inst, double* U);
 /* Primitive state and final reconstruction types of mesh inst; either may be NULL. */

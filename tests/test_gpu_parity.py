@@ -230,6 +230,7 @@ def test_device_clock_matches_host_loop(afar):
     launches = b.run_until(0.05, chunk=16)
     assert launches >= n
     assert b.device_time() == t
+    assert b.device_steps() == n
     assert np.array_equal(a.conserved(), b.conserved())
     b.evolve_auto(5)  # past the end: no change
     assert np.array_equal(a.conserved(), b.conserved())
