@@ -298,8 +298,19 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
 // reductions over the group (the limiter is a minimum, the smoothness a
 // maximum, the trouble test that of the last internal face), so the values
 // equal the one-lane-per-cell kernel's bit for bit.
+// Minimum waves per SIMD the register allocator must allow for the four-lane
+// kernel (0: no constraint); a build-time tuning knob.
+#ifndef AFAR_RECONQ_WPE
+#define AFAR_RECONQ_WPE 0
+#endif
+#if AFAR_RECONQ_WPE > 0
+#define AFAR_RECONQ_ATTR __attribute__((amdgpu_waves_per_eu(AFAR_RECONQ_WPE, 8)))
+#else
+#define AFAR_RECONQ_ATTR
+#endif
+
 template <typename R, int TILE>
-__global__ __launch_bounds__(4 * TILE) void k_reconstruct_q(DevPtrs<R> p) {
+__global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(DevPtrs<R> p) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
     extern __shared__ __align__(16) unsigned char smem_raw[];

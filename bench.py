@@ -63,6 +63,20 @@ def survey_balg_per_cell_step(st, w=8):
     return per_step / nc
 
 
+def measured_traffic(kernel, case, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a copy
+    kernel's reading), or None when there is none."""
+    path = os.path.join(ROOT, "profiles", "riemann1000_%s_hbm_traffic.txt" % precision)
+    if case != "riemann" or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if f and not line.startswith("#") and f[0] == kernel and len(f) == 5:
+            return int(f[4])
+    return None
+
+
 def cpu_baseline(mesh, sc, W0, budget_s=12.0, max_steps=40):
     """The FP64 oracle (OpenMP) on the same mesh: steps until budget_s."""
     from oracle import pyoracle as O
@@ -168,7 +182,8 @@ def main():
     d = kernels[dom]
     balg = survey_balg_per_cell_step(st, w)
     roofline = dict(bound="hbm", achieved=round(d["GBps"], 1), peak=PEAK_HBM_GBPS, unit="GB/s",
-                    frac=round(d["GBps"] / PEAK_HBM_GBPS, 4), traffic=None, kernel=dom,
+                    frac=round(d["GBps"] / PEAK_HBM_GBPS, 4),
+                    traffic=measured_traffic(dom, args.case, args.precision), kernel=dom,
                     alg_bytes_per_launch=int(d["alg_bytes"]), avg_launch_ms=round(d["avg_ms"], 5))
     cpu = None
     if not args.no_cpu:
