@@ -98,6 +98,7 @@ SIGNATURES = {
     "afar_mesh_faces": ([_VP, _I32, _I32, _D, _D, _I32], C.c_int),
     "afar_mesh_polygons": ([_VP, _I32, _D], C.c_int),
     "afar_mesh_precompute": ([_VP, P(Schemes), C.c_char_p], C.c_int),
+    "afar_mesh_precompute_gpu": ([_VP, P(Schemes)], C.c_int),
     "afar_mesh_stencils": ([_VP, _U8, _U8, _I32, _I32], C.c_int),
     "afar_mesh_rbf_basis": ([_VP, _D, _U8, _D], C.c_int),
     "afar_mesh_gauss_points": ([_VP, C.c_int32, C.c_int32, _D, _D, C.c_int32, _I32], C.c_int),

@@ -17,7 +17,8 @@ LIB = os.path.join(HERE, "libafar.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["mesh.cpp", "precompute.cpp", "layout.cpp", "foam_io.cpp", "solver.cpp", "capi.cpp", "kernels.hip"]
+SOURCES = ["mesh.cpp", "precompute.cpp", "layout.cpp", "foam_io.cpp", "solver.cpp", "capi.cpp", "kernels.hip",
+           "precompute_gpu.hip"]
 HOST_ONLY = {"mesh.cpp", "precompute.cpp", "layout.cpp", "foam_io.cpp"}  # no HIP runtime use: plain C++
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result"]

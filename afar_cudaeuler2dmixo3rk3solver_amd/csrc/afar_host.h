@@ -126,8 +126,15 @@ std::vector<int32_t> rcm_order(int n, const std::vector<int32_t>& adj_ptr, const
 Stencils build_stencils(const Mesh& m, int stencilType);
 void gauss_cell(const Mesh& m, int cell, int cOrder, std::vector<Vec2>& pts, std::vector<double>& w);
 void gauss_boundary_face(const Mesh& m, int b, int fOrder, std::vector<Vec2>& pts, std::vector<double>& w);
-void pinv_jacobi(int n, const double* A, double* G);  // row-major n x n
+void pinv_jacobi(int n, const double* A, double* G);  // row-major n x n, n <= 21
+void gauss_tables(const Mesh& m, const Schemes& sc, std::vector<double>& pts, std::vector<double>& w,
+                  std::vector<int64_t>& ptr);
+void rbf_cell_inputs(const Mesh& m, const Stencils& st, const double* tp, const double* tw, const int64_t* tptr, int c,
+                     double* X, std::vector<double>& gp, std::vector<double>& gw, int* gptr);
 Precomputed precompute(const Mesh& m, const Schemes& sc);
+// ---- precompute_gpu.hip: the RBF part of precompute on the GPU, from pc.st
+// (rbf, trouble, refDistance replaced); equal to the host's bit for bit.
+void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc);
 void write_basis_lists(const Mesh& m, const Precomputed& pc, const std::string& dir);
 void read_basis_lists(const Mesh& m, Precomputed& pc, const std::string& dir);
 void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& fn);

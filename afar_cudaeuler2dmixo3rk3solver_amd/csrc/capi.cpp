@@ -339,6 +339,22 @@ int afar_mesh_polygons(const afar_mesh* h, int32_t* vptr, double* vxy) {
 
 // ---------------------------------------------------------------- precompute
 
+static void finish_precompute(afar_mesh* h, const Schemes& sc);
+
+int afar_mesh_precompute_gpu(afar_mesh* h, const afar_schemes* s) {
+    return guard([&] {
+        need(h && s, "NULL argument");
+        Schemes sc = to_schemes(s);
+        need(sc.recalcBasis, "the GPU precompute computes the basis (recalcBasis true)");
+        Schemes st_only = sc;
+        st_only.recalcBasis = 0;  // stencils on the host, coefficients on the GPU
+        Precomputed pc = precompute(h->m, st_only);
+        precompute_rbf_gpu(h->m, sc, pc);
+        h->pc = std::move(pc);
+        finish_precompute(h, sc);
+    });
+}
+
 int afar_mesh_precompute(afar_mesh* h, const afar_schemes* s, const char* basis_dir) {
     return guard([&] {
         need(h && s, "NULL argument");
@@ -358,6 +374,12 @@ int afar_mesh_precompute(afar_mesh* h, const afar_schemes* s, const char* basis_
                 throw Error(AFAR_ERR_IO, e.what());
             }
         }
+        finish_precompute(h, sc);
+    });
+}
+
+static void finish_precompute(afar_mesh* h, const Schemes& sc) {
+    {
         h->sc = sc;
         h->precomputed = true;
         // reference block-table sizes (reported, and used by the seam-2 export)
@@ -373,7 +395,7 @@ int afar_mesh_precompute(afar_mesh* h, const afar_schemes* s, const char* basis_
         BlockTables bt = make_block_tables(ra);
         h->maxLB = bt.maxLB;
         h->maxCLB = bt.maxCLB;
-    });
+    }
 }
 
 int afar_mesh_stencils(const afar_mesh* h, uint8_t* S, uint8_t* C, int32_t* stencil, int32_t* cellFaces) {

@@ -15,6 +15,7 @@
 #include <thread>
 
 #include "afar_host.h"
+#include "rbf_core.h"
 
 namespace afar {
 
@@ -343,70 +344,41 @@ void gauss_boundary_face(const Mesh& m, int b, int fOrder, std::vector<Vec2>& pt
 
 // ------------------------------------------------------------------ pseudo-inverse
 
-// Moore-Penrose pseudo-inverse by one-sided (Hestenes) Jacobi SVD:
-// A V = U S, pinv(A) = V S^+ U^T, with singular values equal to zero dropped
-// (OpenFOAM's SVDinv with minCondition 0, basicCUDAsolver.cpp:509).  Cyclic
-// sweeps in a fixed order make the result a deterministic function of A.
 void pinv_jacobi(int n, const double* A, double* G) {
-    std::vector<double> U(n * n), Vm(n * n, 0.0);
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) U[j * n + i] = A[i * n + j];  // U stored column-major (column j contiguous)
-    for (int i = 0; i < n; ++i) Vm[i * n + i] = 1.0;                // V column-major too
-    for (int sweep = 0; sweep < 80; ++sweep) {
-        bool rotated = false;
-        for (int p = 0; p < n - 1; ++p)
-            for (int q = p + 1; q < n; ++q) {
-                double* up = &U[p * n];
-                double* uq = &U[q * n];
-                double alpha = 0.0, beta = 0.0, gamma = 0.0;
-                for (int k = 0; k < n; ++k) {
-                    alpha += up[k] * up[k];
-                    beta += uq[k] * uq[k];
-                    gamma += up[k] * uq[k];
-                }
-                if (gamma == 0.0 || fabs(gamma) <= 1e-15 * sqrt(alpha * beta)) continue;
-                rotated = true;
-                const double zeta = (beta - alpha) / (2.0 * gamma);
-                double t;
-                if (fabs(zeta) > 1e150)
-                    t = 0.5 / zeta;
-                else
-                    t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                const double c = 1.0 / sqrt(1.0 + t * t), s = c * t;
-                for (int k = 0; k < n; ++k) {
-                    const double a = up[k], b = uq[k];
-                    up[k] = c * a - s * b;
-                    uq[k] = s * a + c * b;
-                }
-                double* vp = &Vm[p * n];
-                double* vq = &Vm[q * n];
-                for (int k = 0; k < n; ++k) {
-                    const double a = vp[k], b = vq[k];
-                    vp[k] = c * a - s * b;
-                    vq[k] = s * a + c * b;
-                }
-            }
-        if (!rotated) break;
-    }
-    std::vector<double> sig(n);
-    for (int j = 0; j < n; ++j) {
-        double s = 0.0;
-        for (int k = 0; k < n; ++k) s += U[j * n + k] * U[j * n + k];
-        sig[j] = sqrt(s);
-    }
-    for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) {
-            double g = 0.0;
-            for (int k = 0; k < n; ++k) {
-                if (!(sig[k] > 0.0)) continue;
-                // V[i][k] * (1/sigma_k) * u_k[j], u_k = U[:,k] / sigma_k
-                g += Vm[k * n + i] * (U[k * n + j] / sig[k]) / sig[k];
-            }
-            G[i * n + j] = g;
-        }
+    if (n < 1 || n > kRbfMaxN) throw Error(1, "pinv_jacobi: matrix size out of range");
+    std::vector<double> U((size_t)n * n), V((size_t)n * n), sig(n);
+    for (int i = 0; i < n * n; ++i) U[i] = A[i];
+    rbf_pinv(n, U.data(), V.data(), sig.data(), G);
 }
 
 // ------------------------------------------------------------------ RBF precompute
+
+// Quadrature of every cell (cOrder rule) and boundary face (fOrder rule) as
+// flat tables: points 2 per entry, weights, and nc+nb+1 offsets.
+void gauss_tables(const Mesh& m, const Schemes& sc, std::vector<double>& pts, std::vector<double>& w,
+                  std::vector<int64_t>& ptr) {
+    const int nc = m.nc, nb = m.nb;
+    std::vector<std::vector<Vec2>> P(nc + nb);
+    std::vector<std::vector<double>> Wt(nc + nb);
+    parallel_for(nc + nb, [&](int64_t i0, int64_t i1) {
+        for (int64_t i = i0; i < i1; ++i) {
+            if (i < nc)
+                gauss_cell(m, (int)i, sc.cOrder, P[i], Wt[i]);
+            else
+                gauss_boundary_face(m, (int)(i - nc), sc.fOrder, P[i], Wt[i]);
+        }
+    });
+    ptr.assign(nc + nb + 1, 0);
+    for (int i = 0; i < nc + nb; ++i) ptr[i + 1] = ptr[i] + (int64_t)P[i].size();
+    pts.resize(2 * (size_t)ptr[nc + nb]);
+    w.resize((size_t)ptr[nc + nb]);
+    for (int i = 0; i < nc + nb; ++i)
+        for (size_t k = 0; k < P[i].size(); ++k) {
+            pts[2 * (ptr[i] + k)] = P[i][k].x;
+            pts[2 * (ptr[i] + k) + 1] = P[i][k].y;
+            w[ptr[i] + k] = Wt[i][k];
+        }
+}
 
 Precomputed precompute(const Mesh& m, const Schemes& sc) {
     Precomputed pc;
@@ -419,62 +391,36 @@ Precomputed precompute(const Mesh& m, const Schemes& sc) {
     pc.refDistance.assign(nc, 0.0);
     if (!sc.recalcBasis) return pc;  // coefficients come from read_basis_lists
     const double sf = sc.shapeFactor;
-    // validate the quadrature orders once (throws on unsupported orders)
-    {
-        std::vector<Vec2> p;
-        std::vector<double> w;
-        for (int c = 0; c < std::min(nc, 1); ++c) gauss_cell(m, c, sc.cOrder, p, w);
-        if (m.nb > 0) gauss_boundary_face(m, 0, sc.fOrder, p, w);
-    }
+    std::vector<double> tp, tw;
+    std::vector<int64_t> tptr;
+    gauss_tables(m, sc, tp, tw, tptr);
     std::atomic<int> troubles{0};
-    std::exception_ptr err = nullptr;
     parallel_for(nc, [&](int64_t c0, int64_t c1) {
-        std::vector<Vec2> X, gp;
-        std::vector<double> gw, P, G, bvec, a;
-        std::vector<std::vector<Vec2>> memPts;
-        std::vector<std::vector<double>> memW;
+        double X[2 * kRbfMaxN], U[kRbfMaxN * kRbfMaxN], V[kRbfMaxN * kRbfMaxN], G[kRbfMaxN * kRbfMaxN];
+        double sig[kRbfMaxN], b[kRbfMaxN], a[kRbfMaxN];
+        int gptr[kRbfMaxN + 1];
+        std::vector<double> gp, gw;
         for (int64_t c = c0; c < c1; ++c) {
             const int S = st.S[c], n = S + 1;
-            X.assign(n, Vec2{});
-            memPts.assign(n, {});
-            memW.assign(n, {});
-            X[0] = m.C[c];
-            gauss_cell(m, (int)c, sc.cOrder, memPts[0], memW[0]);
+            rbf_cell_inputs(m, st, tp.data(), tw.data(), tptr.data(), (int)c, X, gp, gw, gptr);
             double ref = 0.0;
             for (int i = 1; i < n; ++i) {
-                const int sid = st.sten[(size_t)c * kMaxValidStencil + i - 1];
-                const Vec2 sh = st.sten_shift[(size_t)c * kMaxValidStencil + i - 1];
-                X[i] = Vec2{m.C[sid].x + sh.x, m.C[sid].y + sh.y};
-                const double refdx = X[i].x - m.C[c].x, refdy = X[i].y - m.C[c].y;
+                const double refdx = X[2 * i] - m.C[c].x, refdy = X[2 * i + 1] - m.C[c].y;
                 ref += sqrt(refdx * refdx + refdy * refdy);
-                if (sid < nc)
-                    gauss_cell(m, sid, sc.cOrder, memPts[i], memW[i]);
-                else
-                    gauss_boundary_face(m, sid - nc, sc.fOrder, memPts[i], memW[i]);
-                for (auto& q : memPts[i]) {
-                    q.x += sh.x;
-                    q.y += sh.y;
-                }
             }
             ref /= (double)S;
             pc.refDistance[c] = ref;
-            P.assign((size_t)n * n, 0.0);
-            for (int i = 0; i < n; ++i)
-                for (int j = 0; j < n; ++j) {
-                    const double x0 = X[j].x, y0 = X[j].y;
-                    double acc = 0.0;
-                    for (size_t k = 0; k < memPts[i].size(); ++k) {
-                        const double dx = (memPts[i][k].x - x0) / ref, dy = (memPts[i][k].y - y0) / ref;
-                        const double r2 = dx * dx + dy * dy;
-                        acc += memW[i][k] * sqrt(sf + r2);
-                    }
-                    P[(size_t)i * n + j] = acc;
-                }
-            G.assign((size_t)n * n, 0.0);
-            pinv_jacobi(n, P.data(), G.data());
-            bool trouble = false;
-            for (int i = 0; i < n * n && !trouble; ++i)
-                if (fabs(G[i]) > 1e10 || G[i] != G[i]) trouble = true;
+            int count[kRbfMaxN];
+            for (int i = 0; i < n; ++i) count[i] = gptr[i + 1] - gptr[i];
+            rbf_moments(n, X, count,
+                        [&](int i, int k, double& x, double& y, double& w) {
+                            x = gp[2 * (gptr[i] + k)];
+                            y = gp[2 * (gptr[i] + k) + 1];
+                            w = gw[gptr[i] + k];
+                        },
+                        ref, sf, U);
+            rbf_pinv(n, U, V, sig, G);
+            const bool trouble = rbf_trouble(n, G);
             pc.trouble[c] = trouble;
             if (trouble) troubles++;
             // coefficients of every internal face of c on c's side
@@ -492,26 +438,42 @@ Precomputed precompute(const Mesh& m, const Schemes& sc) {
                 // face centre in c's frame
                 Vec2 xf = m.Cf[f];
                 if (side) xf = Vec2{xf.x - m.fshift[f].x, xf.y - m.fshift[f].y};
-                bvec.assign(n, 0.0);
-                for (int i = 0; i < n; ++i) {
-                    const double dx = (xf.x - X[i].x) / ref, dy = (xf.y - X[i].y) / ref;
-                    bvec[i] = sqrt(sf + (dx * dx + dy * dy));
-                }
-                a.assign(n, 0.0);
-                double sum = 0.0;
-                for (int i = 0; i < n; ++i) {
-                    double v = 0.0;
-                    for (int j = 0; j < n; ++j) v += G[(size_t)j * n + i] * bvec[j];
-                    a[i] = v;
-                    sum += v;
-                }
-                for (int i = 0; i < n; ++i) out[i] = (a[i] - sum / (double)n) + 1.0 / (double)n;
+                rbf_face(n, G, X, xf.x, xf.y, ref, sf, b, a);
+                for (int i = 0; i < n; ++i) out[i] = a[i];
             }
         }
     });
-    if (err) std::rethrow_exception(err);
     pc.troubleCount = troubles;
     return pc;
+}
+
+// Member centres (in the cell's frame) and member quadrature of cell c, from
+// the flat quadrature tables (the shift of periodic members applied).
+void rbf_cell_inputs(const Mesh& m, const Stencils& st, const double* tp, const double* tw, const int64_t* tptr, int c,
+                     double* X, std::vector<double>& gp, std::vector<double>& gw, int* gptr) {
+    const int S = st.S[c], n = S + 1;
+    gp.clear();
+    gw.clear();
+    gptr[0] = 0;
+    for (int i = 0; i < n; ++i) {
+        int id = c;
+        Vec2 sh{0.0, 0.0};
+        if (i > 0) {
+            id = st.sten[(size_t)c * kMaxValidStencil + i - 1];
+            sh = st.sten_shift[(size_t)c * kMaxValidStencil + i - 1];
+            X[2 * i] = m.C[id].x + sh.x;
+            X[2 * i + 1] = m.C[id].y + sh.y;
+        } else {
+            X[0] = m.C[c].x;
+            X[1] = m.C[c].y;
+        }
+        for (int64_t k = tptr[id]; k < tptr[id + 1]; ++k) {
+            gp.push_back(i > 0 ? tp[2 * k] + sh.x : tp[2 * k]);
+            gp.push_back(i > 0 ? tp[2 * k + 1] + sh.y : tp[2 * k + 1]);
+            gw.push_back(tw[k]);
+        }
+        gptr[i + 1] = (int)gw.size();
+    }
 }
 
 // ------------------------------------------------------------------ basis cache I/O

@@ -1,0 +1,175 @@
+// RBF precompute on the GPU (basicCUDAsolver.cpp:461-619): one thread per
+// cell assembles the moment matrix from the members' quadrature, inverts it
+// by the Jacobi SVD and writes the coefficients of the cell's side of each of
+// its internal faces.  The arithmetic is rbf_core.h, shared with the host
+// precompute, so the coefficients equal the host's bit for bit.  Stencils,
+// quadrature tables and geometry come from the host; the work matrices (three
+// of at most 21 x 21 doubles per thread) live in private memory.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "afar_host.h"
+#include "rbf_core.h"
+#include "solver.h"
+
+namespace afar {
+
+namespace {
+
+struct RbfArgs {
+    int nc, nf, W;  // W = maxS + 1
+    double sf;
+    const uint8_t* S;
+    const uint8_t* Cc;
+    const int32_t* sten;        // nc * kMaxValidStencil
+    const double2* sten_shift;  // nc * kMaxValidStencil
+    const int32_t* cellFaces;   // nc * kMaxValidStencil
+    const double2* C;           // nc + nb centres
+    const double* tp;           // quadrature points, 2 per entry
+    const double* tw;           // quadrature weights
+    const int64_t* tptr;        // nc + nb + 1
+    const double2* Cf;          // nf face centres
+    const double2* fshift;      // nf
+    double* rbf;                // 2 * W * nf
+    uint8_t* trouble;           // nc
+    double* ref;                // nc
+};
+
+// Few waves per SIMD: each thread holds about 11 KB of private work arrays.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_rbf_precompute(RbfArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.nc) return;
+    const int S = a.S[c], n = S + 1;
+    double X[2 * kRbfMaxN], U[kRbfMaxN * kRbfMaxN], V[kRbfMaxN * kRbfMaxN], G[kRbfMaxN * kRbfMaxN];
+    double sig[kRbfMaxN], b[kRbfMaxN], out[kRbfMaxN];
+    int id[kRbfMaxN], count[kRbfMaxN];
+    double shx[kRbfMaxN], shy[kRbfMaxN];
+    X[0] = a.C[c].x;
+    X[1] = a.C[c].y;
+    id[0] = c;
+    shx[0] = shy[0] = 0.0;
+    for (int i = 1; i < n; ++i) {
+        const size_t s = (size_t)c * kMaxValidStencil + i - 1;
+        id[i] = a.sten[s];
+        shx[i] = a.sten_shift[s].x;
+        shy[i] = a.sten_shift[s].y;
+        X[2 * i] = a.C[id[i]].x + shx[i];
+        X[2 * i + 1] = a.C[id[i]].y + shy[i];
+    }
+    for (int i = 0; i < n; ++i) count[i] = (int)(a.tptr[id[i] + 1] - a.tptr[id[i]]);
+    double ref = 0.0;
+    for (int i = 1; i < n; ++i) {
+        const double refdx = X[2 * i] - a.C[c].x, refdy = X[2 * i + 1] - a.C[c].y;
+        ref += sqrt(refdx * refdx + refdy * refdy);
+    }
+    ref /= (double)S;
+    a.ref[c] = ref;
+    rbf_moments(n, X, count,
+                [&](int i, int k, double& x, double& y, double& w) {
+                    const int64_t e = a.tptr[id[i]] + k;
+                    x = i > 0 ? a.tp[2 * e] + shx[i] : a.tp[2 * e];
+                    y = i > 0 ? a.tp[2 * e + 1] + shy[i] : a.tp[2 * e + 1];
+                    w = a.tw[e];
+                },
+                ref, a.sf, U);
+    rbf_pinv(n, U, V, sig, G);
+    const bool trouble = rbf_trouble(n, G);
+    a.trouble[c] = trouble;
+    for (int fi = 0; fi < a.Cc[c]; ++fi) {
+        const int sfc = a.cellFaces[(size_t)c * kMaxValidStencil + fi];
+        const int f = (sfc > 0 ? sfc : -sfc) - 1;
+        if (f >= a.nf) continue;
+        const int side = sfc < 0;
+        double* o = a.rbf + (size_t)2 * a.W * f + (size_t)side * a.W;
+        if (trouble) {
+            o[0] = 1.0;
+            for (int i = 1; i < n; ++i) o[i] = 0.0;
+            continue;
+        }
+        double xf = a.Cf[f].x, yf = a.Cf[f].y;
+        if (side) {
+            xf = xf - a.fshift[f].x;
+            yf = yf - a.fshift[f].y;
+        }
+        rbf_face(n, G, X, xf, yf, ref, a.sf, b, out);
+        for (int i = 0; i < n; ++i) o[i] = out[i];
+    }
+}
+
+struct Buf {
+    void* p = nullptr;
+    Buf() = default;
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    template <typename T> T* put(const T* src, size_t n) {
+        check_hip(hipMalloc(&p, sizeof(T) * (n ? n : 1)), "hipMalloc");
+        if (n && src) check_hip(hipMemcpy(p, src, sizeof(T) * n, hipMemcpyHostToDevice), "upload");
+        return static_cast<T*>(p);
+    }
+};
+
+}  // namespace
+
+void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
+    const Stencils& st = pc.st;
+    const int nc = m.nc, nf = m.nf, nb = m.nb, W = st.maxS + 1;
+    if (W > kRbfMaxN) throw Error(3, "stencil larger than the RBF work arrays");
+    std::vector<double> tp, tw;
+    std::vector<int64_t> tptr;
+    gauss_tables(m, sc, tp, tw, tptr);
+    std::vector<double2> C(nc + nb), Cf(nf), fsh(nf), shift((size_t)nc * kMaxValidStencil);
+    for (int i = 0; i < nc + nb; ++i) C[i] = double2{m.C[i].x, m.C[i].y};
+    for (int f = 0; f < nf; ++f) {
+        Cf[f] = double2{m.Cf[f].x, m.Cf[f].y};
+        fsh[f] = double2{m.fshift[f].x, m.fshift[f].y};
+    }
+    for (size_t i = 0; i < shift.size(); ++i) shift[i] = double2{st.sten_shift[i].x, st.sten_shift[i].y};
+    Buf bS, bCc, bSten, bShift, bCF, bC, bTp, bTw, bTptr, bCf, bFsh, bRbf, bTr, bRef;
+    RbfArgs a{};
+    a.nc = nc;
+    a.nf = nf;
+    a.W = W;
+    a.sf = sc.shapeFactor;
+    a.S = bS.put(st.S.data(), st.S.size());
+    a.Cc = bCc.put(st.Cc.data(), st.Cc.size());
+    a.sten = bSten.put(st.sten.data(), st.sten.size());
+    a.sten_shift = bShift.put(shift.data(), shift.size());
+    a.cellFaces = bCF.put(st.cellFaces.data(), st.cellFaces.size());
+    a.C = bC.put(C.data(), C.size());
+    a.tp = bTp.put(tp.data(), tp.size());
+    a.tw = bTw.put(tw.data(), tw.size());
+    a.tptr = bTptr.put(tptr.data(), tptr.size());
+    a.Cf = bCf.put(Cf.data(), Cf.size());
+    a.fshift = bFsh.put(fsh.data(), fsh.size());
+    a.rbf = bRbf.put<double>(nullptr, (size_t)2 * W * nf);
+    a.trouble = bTr.put<uint8_t>(nullptr, nc);
+    a.ref = bRef.put<double>(nullptr, nc);
+    // own stream: waits for this work only, not for other solvers on the device
+    hipStream_t s = nullptr;
+    check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+    struct StreamGuard {
+        hipStream_t s;
+        ~StreamGuard() { (void)hipStreamDestroy(s); }
+    } guard{s};
+    check_hip(hipMemsetAsync(a.rbf, 0, sizeof(double) * 2 * W * (size_t)nf, s), "hipMemsetAsync");
+    hipLaunchKernelGGL(k_rbf_precompute, dim3((nc + 63) / 64), dim3(64), 0, s, a);
+    check_hip(hipGetLastError(), "k_rbf_precompute");
+    pc.rbf.assign((size_t)2 * W * nf, 0.0);
+    pc.trouble.assign(nc, 0);
+    pc.refDistance.assign(nc, 0.0);
+    check_hip(hipMemcpyAsync(pc.rbf.data(), a.rbf, sizeof(double) * pc.rbf.size(), hipMemcpyDeviceToHost, s),
+              "download");
+    check_hip(hipMemcpyAsync(pc.trouble.data(), a.trouble, nc, hipMemcpyDeviceToHost, s), "download");
+    check_hip(hipMemcpyAsync(pc.refDistance.data(), a.ref, sizeof(double) * nc, hipMemcpyDeviceToHost, s),
+              "download");
+    check_hip(hipStreamSynchronize(s), "k_rbf_precompute");
+    int t = 0;
+    for (uint8_t v : pc.trouble) t += v;
+    pc.troubleCount = t;
+}
+
+}  // namespace afar

@@ -71,8 +71,9 @@ def request_stream(rate=20.0, duration=120.0, seed=3, nc_min=1e4, nc_max=2e6, t_
     return out
 
 
-def build_case(req, sc):
-    """Mesh (precomputed) and initial state of a request."""
+def build_case(req, sc=None):
+    """Mesh and initial state of a request; precomputed on the host when
+    `sc` is given."""
     if req.kind == "corner":
         ny = max(4, int(round(math.sqrt(req.nc_target / 2.0))))
         nx = max(8, int(round(req.nc_target / ny)))
@@ -81,7 +82,8 @@ def build_case(req, sc):
         # a quad split in two with probability 0.5 gives 1.5 cells per quad
         n = max(8, int(round(math.sqrt(req.nc_target / 1.5))))
         mesh, W0 = cases.riemann_case(n=n, seed=req.seed % 1000)
-    mesh.precompute(sc)
+    if sc is not None:
+        mesh.precompute(sc)
     return mesh, W0
 
 
@@ -110,8 +112,9 @@ class Server:
     solver destroyed, memory and slot released)."""
 
     def __init__(self, precision="mixed", hbm_budget=64 << 30, max_active=8, chunk=50, prep_workers=2,
-                 log=None, poll_s=2e-4):
+                 log=None, poll_s=2e-4, gpu_precompute=True):
         self.precision = precision
+        self.gpu_precompute = gpu_precompute
         self.budget = hbm_budget
         self.max_active = max_active
         self.chunk = chunk
@@ -160,6 +163,8 @@ class Server:
 
     def solve(self, req, mesh, W0, t0):
         """One request alone, with the same chunking as in the loop."""
+        if not mesh.precomputed:
+            mesh.precompute(self.schemes_of(req), gpu=self.gpu_precompute)
         g = self._start(req, mesh, W0, t0)
         while True:
             g.synchronize()
@@ -189,9 +194,11 @@ class Server:
                     ready_q.put(None)
                     return
                 r.times["prep_start"] = time.perf_counter() - t0
-                mesh, W0 = build_case(r, self.schemes_of(r))
+                # with the GPU precompute only the mesh is built here (no GPU
+                # calls on this thread); the scheduler precomputes on admission
+                mesh, W0 = build_case(r, None if self.gpu_precompute else self.schemes_of(r))
                 r.times["prepared"] = time.perf_counter() - t0
-                ready_q.put((r, mesh, W0, footprint_bytes(mesh, self.precision)))
+                ready_q.put([r, mesh, W0, None if self.gpu_precompute else footprint_bytes(mesh, self.precision)])
 
         feeder = threading.Thread(target=arrivals, daemon=True)
         preps = [threading.Thread(target=preparer, daemon=True) for _ in range(self.prep_workers)]
@@ -215,7 +222,13 @@ class Server:
                     producers -= 1
                 else:
                     waiting.append(item)
-            while waiting and len(active) < self.max_active and (used + waiting[0][3] <= self.budget or not active):
+            while waiting and len(active) < self.max_active:
+                if waiting[0][3] is None:  # precompute on the GPU, then size it
+                    waiting[0][1].precompute(self.schemes_of(waiting[0][0]), gpu=True)
+                    waiting[0][3] = footprint_bytes(waiting[0][1], self.precision)
+                    waiting[0][0].times["precomputed"] = time.perf_counter() - t0
+                if used + waiting[0][3] > self.budget and active:
+                    break
                 r, mesh, W0, need = waiting.pop(0)
                 r.times["admitted"] = time.perf_counter() - t0
                 g = self._start(r, mesh, W0, t0)
@@ -256,6 +269,8 @@ def summary_row(r):
     tts = r.times["finish"] - r.arrival
     return dict(rid=r.rid, kind=r.kind, cells=r.cells, steps=r.steps, arrival=round(r.arrival, 3),
                 tts_s=round(tts, 3), prep_s=round(r.times["prepared"] - r.times["prep_start"], 3),
+                gpu_precompute_s=round(r.times["precomputed"] - r.times["prepared"], 3)
+                if "precomputed" in r.times else None,
                 queue_s=round(r.times["admitted"] - r.times["prepared"], 3),
                 gpu_s=round(r.times["finish"] - r.times["start"], 3),
                 residual=None if math.isnan(r.residual) else r.residual)
@@ -265,7 +280,7 @@ def fp64_reference(req, chunk=50):
     """The request solved alone on the all-FP64 path (the FP64 GPU path is
     pinned to the CPU oracle elsewhere), with the same stepping."""
     srv = Server(precision="fp64", max_active=1, chunk=chunk)
-    mesh, W0 = build_case(req, srv.schemes_of(req))
+    mesh, W0 = build_case(req)
     r = Request(req.rid, 0.0, req.kind, req.nc_target, req.seed, req.t_end, req.res_tol, req.max_steps)
     srv.solve(r, mesh, W0, time.perf_counter())
     return r
@@ -285,10 +300,12 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--sample", type=float, default=0.05, help="fraction checked against FP64")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--host-precompute", action="store_true", help="RBF precompute on the host threads")
     args = ap.parse_args(argv)
     reqs = request_stream(args.rate, args.duration, args.seed, args.nc_min, args.nc_max, args.t_end, args.res_tol,
                           args.max_steps)
-    srv = Server(max_active=args.max_active, chunk=args.chunk, log=lambda m: print(m, flush=True))
+    srv = Server(max_active=args.max_active, chunk=args.chunk, log=lambda m: print(m, flush=True),
+                 gpu_precompute=not args.host_precompute)
     done, wall = srv.run(reqs)
     tts = np.array([r.times["finish"] - r.arrival for r in done])
     cell_steps = float(sum(r.cells * r.steps for r in done))

@@ -177,8 +177,16 @@ class Mesh:
         return vptr, vxy
 
     # -- precompute (basicCUDAsolver::initStencilData)
-    def precompute(self, sc, basis_dir=None):
-        check(lib().afar_mesh_precompute(self._h, C.byref(sc), None if basis_dir is None else basis_dir.encode()))
+    def precompute(self, sc, basis_dir=None, gpu=False):
+        """initStencilData: stencils, quadrature, RBF basis (gpu: the RBF part
+        on the GPU, bitwise equal to the host's)."""
+        if gpu:
+            if basis_dir is not None:
+                raise ValueError("the GPU precompute does not read or write the basis cache")
+            check(lib().afar_mesh_precompute_gpu(self._h, C.byref(sc)))
+        else:
+            check(lib().afar_mesh_precompute(self._h, C.byref(sc),
+                                             None if basis_dir is None else basis_dir.encode()))
         self.precomputed = True
         return self
 

@@ -144,6 +144,12 @@ int afar_mesh_polygons(const afar_mesh* m, int32_t* vptr, double* vxy);
  * coefficients are written there as basisOwnList/basisNeiList; when
  * recalcBasis is 0 they are read from there (basicCUDAsolver.cpp:626-694). */
 int afar_mesh_precompute(afar_mesh* m, const afar_schemes* s, const char* basis_dir);
+/* Same precompute with the RBF moment matrices, their pseudo-inverses and the
+ * face coefficients computed on the GPU (one thread per cell); equal to
+ * afar_mesh_precompute bit for bit (shared arithmetic, IEEE double).
+ * Extension without a reference counterpart, for meshes whose host
+ * precompute dominates the time to solution. */
+int afar_mesh_precompute_gpu(afar_mesh* m, const afar_schemes* s);
 /* S nc, C nc, stencil maxS*nc (global ids, pseudo-cells >= nc), cellFaces maxC*nc (+-(face+1)). */
 int afar_mesh_stencils(const afar_mesh* m, uint8_t* S, uint8_t* C, int32_t* stencil, int32_t* cellFaces);
 /* rbf 2*(maxS+1)*nf (RBFbasis layout), trouble nc, refDistance nc; NULL to skip. */

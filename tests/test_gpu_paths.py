@@ -108,3 +108,25 @@ def test_restart_is_exact(afar):
     b.set_conserved(U10)
     b.evolve_auto(10)
     assert np.array_equal(a.conserved(), b.conserved())
+
+
+@pytest.mark.parametrize("kind", ["corner", "riemann", "periodic"])
+def test_gpu_precompute_equals_host(afar, kind):
+    """The RBF precompute on the GPU gives the host's coefficients, trouble
+    flags and reference distances bit for bit."""
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    say("case: GPU precompute vs host, %s" % kind)
+    mk = {"corner": lambda: cases.corner_case(nx=60, ny=30)[0],
+          "riemann": lambda: cases.riemann_case(n=48, seed=4)[0],
+          "periodic": lambda: afar.Mesh.periodic(n=24, split_p=0.5, seed=2)}[kind]
+    a, b = mk(), mk()
+    sc = afar.schemes(cOrder=7, fOrder=2)
+    a.precompute(sc)
+    b.precompute(sc, gpu=True)
+    ra, ta, da = a.rbf_basis()
+    rb, tb, db = b.rbf_basis()
+    assert np.array_equal(ra, rb)
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(da, db)
+    for x, y in zip(a.stencils(), b.stencils()):
+        assert np.array_equal(x, y)
