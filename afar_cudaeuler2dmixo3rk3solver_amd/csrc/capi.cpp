@@ -529,7 +529,8 @@ int afar_solver_create(const afar_mesh* h, const afar_schemes* s, const double* 
         cfg.exactRK = s->exactRK;
         cfg.tile = tile_of(s);
         cfg.boundaryFirstOrder = s->boundaryFirstOrder != 0;
-        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 4, "reconLanes must be 0, 1 or 4");
+        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 2 || s->reconLanes == 4,
+             "reconLanes must be 0, 1, 2 or 4");
         cfg.reconLanes = s->reconLanes;
         auto sol = std::make_unique<afar_solver>();
         sol->dev = make_device_solver(ra, cfg);
@@ -567,7 +568,8 @@ int afar_batch_create(int32_t n, const afar_mesh* const* meshes, const afar_sche
         cfg.exactRK = s->exactRK;
         cfg.tile = tile_of(s);
         cfg.boundaryFirstOrder = s->boundaryFirstOrder != 0;
-        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 4, "reconLanes must be 0, 1 or 4");
+        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 2 || s->reconLanes == 4,
+             "reconLanes must be 0, 1, 2 or 4");
         cfg.reconLanes = s->reconLanes;
         b->dev = make_device_solver(u, cfg, &b->idx);
         *out = b.release();

@@ -8,6 +8,7 @@
 namespace afar {
 
 constexpr int kCflBlocks = 1024;
+constexpr int kMaxSten = 20;  // largest stencil (the reference's limit, basicCUDAsolver.cpp:393-397)
 
 template <typename R> struct VT;
 template <> struct VT<double> {
@@ -72,8 +73,9 @@ struct DevPtrs {
 
 template <typename R> void launch_lsq(const DevPtrs<R>& p, hipStream_t s);
 template <typename R> void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s);
-// lanes: 1 = one thread per cell (any mesh); 4 = four threads per cell
-// (meshes whose cells have at most four faces).
+// lanes: 1 = one thread per cell (any mesh); 4 = four threads per cell;
+// 2 = four threads per cell with the face work compacted (both: meshes whose
+// cells have at most four faces; 2 also tiles of at most 128 cells).
 template <typename R> void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s, int lanes);
 template <typename R> void launch_bvd(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s);
 template <typename R> void launch_flux(const DevPtrs<R>& p, double Rg, double Cv, hipStream_t s);
@@ -91,6 +93,7 @@ template <typename R> void launch_residual_norm(const DevPtrs<R>& p, double* out
 template <typename R>
 void launch_update(const DevPtrs<R>& p, double Rg, double gamma, double dt, const double* dt_ptr, double b1, double b2,
                    double b3, int last, hipStream_t s);
+// Raises each reconstruction kernel's dynamic-LDS limit to lds minus its static LDS.
 template <typename R> hipError_t set_lds_limits(size_t lds);
 void launch_math_selftest(const double* x, double* out, int n, hipStream_t s);
 

@@ -451,6 +451,207 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     }
 }
 
+// K5 with the face work of a tile compacted: for meshes whose cells have at
+// most four faces.  4*TILE threads per tile.
+//  phase 1  four lanes per cell: gradient and compact min/max (every lane of
+//           the group), the flat branch (its face values by lane k), and for
+//           cells on the full path the per-cell data in LDS and one work item
+//           per internal face, counted into buckets by stencil size;
+//  phase 2  the items, ordered by stencil size so a wavefront's RBF loops
+//           have similar trip counts, spread over all threads: RBF face
+//           value, limiter candidates, smoothness term, trouble test;
+//  phase 3  one lane per full-path cell reduces its faces' results in slot
+//           order (min for the limiter, max for the smoothness, the last
+//           internal face's trouble test), exactly as the sequential loop.
+// The per-face arithmetic is that of k_reconstruct, so every output equals
+// the one-lane kernel's bit for bit.  Flat and full cells, triangles and
+// quads no longer idle each other's lanes.
+template <typename R, int TILE>
+__global__ __launch_bounds__(4 * TILE) void k_reconstruct_c(DevPtrs<R> p) {
+    using v4 = typename VT<R>::v4;
+    using v2 = typename VT<R>::v2;
+    constexpr int NT = 4 * TILE, MAXC = 4, NB = kMaxSten + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    v4* sW = reinterpret_cast<v4*>(smem_raw);
+    uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
+    // per-cell data of full-path cells: gradient (8), min/max (8)
+    __shared__ R sCell[TILE][16];
+    // per-slot results: limiter candidates (4), smoothness term, trouble test
+    __shared__ R sRes[TILE * MAXC][5];
+    __shared__ uint8_t sNT[TILE * MAXC];
+    __shared__ uint8_t sLive[TILE * MAXC];
+    __shared__ uint8_t sFull[TILE];
+    __shared__ uint16_t sItem[TILE * MAXC];
+    __shared__ int sBucket[NB + 1];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    if (tile >= p.ntiles) return;
+    const int t = threadIdx.x;
+    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
+    if (t < TILE && tile * TILE + t < p.nc) sW[t] = p.W[tile * TILE + t];
+    for (int i = t; i < hn; i += NT) sW[TILE + i] = p.W[p.halo[hb + i]];
+    stage_stencil(sSt, p.sten, tile, TILE, p.nc, p.maxS, t, NT);
+    for (int i = t; i <= NB; i += NT) sBucket[i] = 0;
+    __syncthreads();
+
+    const int maxS = p.maxS, maxC = p.maxC, KR = maxC * (maxS + 1), nf = p.nf;
+    // ---- phase 1
+    const int lc = t >> 2, k = t & 3;
+    const int c = tile * TILE + lc;
+    const bool valid = c < p.nc;
+    int S = 0;
+    bool full = false, live = false;
+    if (valid) {
+        S = p.scnt[c];
+        const int C = p.ccnt[c];
+        const v4 wc = sW[lc];
+        R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
+        for (int i = 0; i < C; ++i) {
+            const v2 M = p.lsq[sell(c, i, maxC)];
+            const v4 wj = sW[sSt[sell(lc, i, maxS)]];
+            R dphi = wj.x - wc.x;
+            gpx += M.x * dphi;
+            gpy += M.y * dphi;
+            dphi = wj.y - wc.y;
+            guxx += M.x * dphi;
+            guyx += M.y * dphi;
+            dphi = wj.z - wc.z;
+            guxy += M.x * dphi;
+            guyy += M.y * dphi;
+            dphi = wj.w - wc.w;
+            gTx += M.x * dphi;
+            gTy += M.y * dphi;
+        }
+        if (k == 0) p.gp[c] = v2{gpx, gpy};
+        if (k == 1) p.gu[c] = v2{guxx, guyx};
+        if (k == 2) p.gv[c] = v2{guxy, guyy};
+        if (k == 3) p.gT[c] = v2{gTx, gTy};
+        R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z, TMax = wc.w,
+          TMin = wc.w;
+        for (int i = 0; i < C; ++i) {
+            const v4 wj = sW[sSt[sell(lc, i, maxS)]];
+            pMax = fmax(wj.x, pMax);
+            pMin = fmin(wj.x, pMin);
+            uxMax = fmax(wj.y, uxMax);
+            uxMin = fmin(wj.y, uxMin);
+            uyMax = fmax(wj.z, uyMax);
+            uyMin = fmin(wj.z, uyMin);
+            TMax = fmax(wj.w, TMax);
+            TMin = fmin(wj.w, TMin);
+        }
+        const int sf = k < C ? p.slot_face[sell(c, k, maxC)] : 0;
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        live = k < C && f < nf;
+        full = !(fabs(TMax - TMin) < K<R>::VSMALL);
+        if (!full) {
+            if (k == 0) {
+                p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
+                p.smooth[c] = 0;
+            }
+            if (live) {
+                const double2 d = p.slot_d[sell(c, k, maxC)];
+                const R dx = (R)d.x, dy = (R)d.y;
+                v4 o;
+                o.x = wc.x + gpx * dx + gpy * dy;
+                o.y = wc.y + guxx * dx + guyx * dy;
+                o.z = wc.z + guxy * dx + guyy * dy;
+                o.w = wc.w + gTx * dx + gTy * dy;
+                const int isNei = sf < 0;
+                p.fv[2 * f + isNei] = o;
+                p.fvT[2 * f + isNei] = o.w;
+            }
+        } else if (k == 0) {
+            R* d = sCell[lc];
+            d[0] = gpx, d[1] = gpy, d[2] = guxx, d[3] = guyx, d[4] = guxy, d[5] = guyy, d[6] = gTx, d[7] = gTy;
+            d[8] = pMax, d[9] = pMin, d[10] = uxMax, d[11] = uxMin, d[12] = uyMax, d[13] = uyMin, d[14] = TMax;
+            d[15] = TMin;
+        }
+        if (k == 0) sFull[lc] = full;
+        sLive[lc * MAXC + k] = live;
+        if (full && live) atomicAdd(&sBucket[S + 1], 1);
+    } else if (k == 0) {
+        sFull[lc] = 0;
+    }
+    if (!valid) sLive[lc * MAXC + k] = 0;
+    __syncthreads();
+    if (t == 0)  // bucket starts: items ordered by stencil size
+        for (int i = 1; i <= NB; ++i) sBucket[i] += sBucket[i - 1];
+    __syncthreads();
+    if (full && live) sItem[atomicAdd(&sBucket[S], 1)] = (uint16_t)((lc << 2) | k);
+    __syncthreads();
+    const int nItems = sBucket[NB];
+
+    // ---- phase 2
+    for (int it = t; it < nItems; it += NT) {
+        const int e = sItem[it], lci = e >> 2, ki = e & 3;
+        const int ci = tile * TILE + lci;
+        const int Si = p.scnt[ci];
+        const v4 wc = sW[lci];
+        const R* d = sCell[lci];
+        const int sf = p.slot_face[sell(ci, ki, maxC)];
+        const int isNei = sf < 0;
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        const double2 dd = p.slot_d[sell(ci, ki, maxC)];
+        const R dx = (R)dd.x, dy = (R)dd.y;
+        const double2 ee = p.slot_e[sell(ci, ki, maxC)];
+        const R deltaR = (R)afar_hypot(ee.x, ee.y);
+        const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
+        const int nbl = sSt[sell(lci, ki, maxS)];
+        const R* a = p.rbf + sell(ci, ki * (maxS + 1), KR);
+        R coef = a[0];
+        R pf = coef * wc.x, uxf = coef * wc.y, uyf = coef * wc.z, Tf = coef * wc.w;
+        for (int m = 0; m < Si; ++m) {
+            const v4 wj = sW[sSt[sell(lci, m, maxS)]];
+            coef = a[(size_t)(m + 1) * 64];
+            pf += coef * wj.x;
+            uxf += coef * wj.y;
+            uyf += coef * wj.z;
+            Tf += coef * wj.w;
+        }
+        const R rP = fabs(d[8] - d[9]), rUx = fabs(d[10] - d[11]), rUy = fabs(d[12] - d[13]), rT = fabs(d[14] - d[15]);
+        R limP = 2, limUx = 2, limUy = 2, limT = 2;
+        v4 o;
+        R phif;
+        limiter_process(limP, phif, wc.x, d[8], d[9], d[0], d[1], dx, dy);
+        o.x = (R)(rP >= K<R>::SMALL) * pf + (R)(rP < K<R>::SMALL) * phif;
+        const R sm = fabs(((R)1 - fraction) * wc.x + fraction * sW[nbl].x - phif) / deltaR +
+                     (R)(pf <= (R)0) * (R)1e20 + (R)(phif <= (R)0) * (R)1e20;
+        limiter_process(limUx, phif, wc.y, d[10], d[11], d[2], d[3], dx, dy);
+        o.y = (R)(rUx >= K<R>::SMALL) * uxf + (R)(rUx < K<R>::SMALL) * phif;
+        limiter_process(limUy, phif, wc.z, d[12], d[13], d[4], d[5], dx, dy);
+        o.z = (R)(rUy >= K<R>::SMALL) * uyf + (R)(rUy < K<R>::SMALL) * phif;
+        limiter_process(limT, phif, wc.w, d[14], d[15], d[6], d[7], dx, dy);
+        o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
+        p.fv[2 * f + isNei] = o;
+        p.fvT[2 * f + isNei] = o.w;
+        R* r = sRes[lci * MAXC + ki];
+        r[0] = limP, r[1] = limUx, r[2] = limUy, r[3] = limT, r[4] = sm;
+        sNT[lci * MAXC + ki] = phif > (R)0;
+    }
+    __syncthreads();
+
+    // ---- phase 3: per-cell reductions in slot order
+    if (t < TILE && sFull[t]) {
+        const int cc = tile * TILE + t;
+        R limP = 2, limUx = 2, limUy = 2, limT = 2, smoothness = 0;
+        int isNotTrouble = 1;
+        for (int s = 0; s < MAXC; ++s) {
+            if (!sLive[t * MAXC + s]) continue;
+            const R* r = sRes[t * MAXC + s];
+            limP = fmin(limP, r[0]);
+            limUx = fmin(limUx, r[1]);
+            limUy = fmin(limUy, r[2]);
+            limT = fmin(limT, r[3]);
+            smoothness = fmax(smoothness, r[4]);
+            isNotTrouble = sNT[t * MAXC + s];
+        }
+        const R nt = (R)isNotTrouble;
+        const R* d = sCell[t];
+        p.lim[cc] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
+        smoothness = smoothness / ((R)0.5 * (d[8] + d[9]) * nt + K<R>::VSMALL);
+        p.smooth[cc] = smoothness > (R)1;
+    }
+}
+
 // ---------------------------------------------------------------- K6 BVD indicator
 
 template <typename R> __device__ __forceinline__ R to_bj(R y) { return fmin((R)2 * y, (R)1); }
@@ -881,6 +1082,13 @@ void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream
 template <typename R>
 void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s, int lanes) {
     const int nb = pad8(p.ntiles);
+    if (lanes == 2) {
+        if (tile == 64)
+            hipLaunchKernelGGL((k_reconstruct_c<R, 64>), dim3(nb), dim3(256), lds, s, p);
+        else
+            hipLaunchKernelGGL((k_reconstruct_c<R, 128>), dim3(nb), dim3(512), lds, s, p);
+        return;
+    }
     if (lanes == 4) {
         switch (tile) {
             case 64: hipLaunchKernelGGL((k_reconstruct_q<R, 64>), dim3(nb), dim3(256), lds, s, p); break;
@@ -945,9 +1153,15 @@ hipError_t set_lds_limits(size_t lds) {
     hipError_t e = hipSuccess;
     const void* fns[] = {(const void*)k_reconstruct<R, 64>,   (const void*)k_reconstruct<R, 128>,
                          (const void*)k_reconstruct<R, 256>,  (const void*)k_reconstruct_q<R, 64>,
-                         (const void*)k_reconstruct_q<R, 128>, (const void*)k_reconstruct_q<R, 256>};
+                         (const void*)k_reconstruct_q<R, 128>, (const void*)k_reconstruct_q<R, 256>,
+                         (const void*)k_reconstruct_c<R, 64>, (const void*)k_reconstruct_c<R, 128>};
     for (const void* fn : fns) {
-        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        // the dynamic limit plus the kernel's static LDS may not exceed the block limit
+        hipFuncAttributes attr;
+        e = hipFuncGetAttributes(&attr, fn);
+        if (e != hipSuccess) return e;
+        const size_t dyn = lds > attr.sharedSizeBytes ? lds - attr.sharedSizeBytes : 0;
+        e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn);
         if (e != hipSuccess) return e;
     }
     return e;

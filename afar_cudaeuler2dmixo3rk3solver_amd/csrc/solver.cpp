@@ -58,17 +58,27 @@ class SolverImpl final : public DeviceSolver {
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         // LDS of the reconstruction: tile + halo states, then the tile's stencil ids
         lds_recon_ = (size_t)(L_.tile + L_.maxHalo) * sizeof(v4) + (size_t)L_.tile * L_.maxS * sizeof(uint16_t);
-        if (cfg.reconLanes == 4 && L_.maxC > 4) throw Error(1, "reconLanes 4 needs cells with at most four faces");
-        lanes_ = cfg.reconLanes == 1 ? 1 : (L_.maxC <= 4 ? 4 : 1);
+        if ((cfg.reconLanes == 4 || cfg.reconLanes == 2) && L_.maxC > 4)
+            throw Error(1, "reconLanes 2 and 4 need cells with at most four faces");
+        if (cfg.reconLanes == 2 && L_.tile > 128) throw Error(1, "reconLanes 2 needs a tile of at most 128 cells");
+        if (cfg.reconLanes != 0)
+            lanes_ = cfg.reconLanes;
+        else
+            lanes_ = L_.maxC > 4 ? 1 : 4;  // four lanes measured fastest at 64 and 128-cell tiles
         lds_bvd_ = (size_t)(L_.tile + L_.maxHalo);
         int dev = 0;
         check_hip(hipGetDevice(&dev), "hipGetDevice");
         hipDeviceProp_t prop;
         check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
         size_t lds_cap = prop.sharedMemPerBlock;
-        if (lds_recon_ > lds_cap)
-            throw Error(3, "Too large local block stencil size: reconstruct needs " + std::to_string(lds_recon_) +
-                               " bytes of LDS, the device allows " + std::to_string(lds_cap));
+        // the compacted kernel adds fixed per-tile arrays (k_reconstruct_c)
+        const size_t lds_static = lanes_ == 2 ? (size_t)L_.tile * (16 + 20) * sizeof(R) + (size_t)L_.tile * 13 +
+                                                    (kMaxSten + 2) * sizeof(int)
+                                              : 0;
+        if (lds_recon_ + lds_static > lds_cap)
+            throw Error(3, "Too large local block stencil size: reconstruct needs " +
+                               std::to_string(lds_recon_ + lds_static) + " bytes of LDS, the device allows " +
+                               std::to_string(lds_cap));
         // The reconstruction kernels' dynamic-LDS limit is raised to the
         // device's per-block maximum once per process: the attribute is
         // global per kernel, so it is never changed while other solvers (on

@@ -17,7 +17,7 @@ struct SolverConfig {
     int exactRK = 0;
     int tile = 64;
     int boundaryFirstOrder = 0;
-    int reconLanes = 0;  // reconstruction threads per cell: 0 automatic, 1, or 4
+    int reconLanes = 0;  // reconstruction kernel: 0 automatic, 1 one lane per cell, 2 compacted, 4 four lanes
 };
 
 // kernel kinds reported by profile()

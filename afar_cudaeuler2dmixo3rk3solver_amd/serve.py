@@ -112,7 +112,7 @@ class Server:
     solver destroyed, memory and slot released)."""
 
     def __init__(self, precision="mixed", hbm_budget=64 << 30, max_active=8, chunk=50, prep_workers=2,
-                 log=None, poll_s=2e-4, gpu_precompute=True):
+                 log=None, poll_s=2e-4, gpu_precompute=False):
         self.precision = precision
         self.gpu_precompute = gpu_precompute
         self.budget = hbm_budget
@@ -269,8 +269,6 @@ def summary_row(r):
     tts = r.times["finish"] - r.arrival
     return dict(rid=r.rid, kind=r.kind, cells=r.cells, steps=r.steps, arrival=round(r.arrival, 3),
                 tts_s=round(tts, 3), prep_s=round(r.times["prepared"] - r.times["prep_start"], 3),
-                gpu_precompute_s=round(r.times["precomputed"] - r.times["prepared"], 3)
-                if "precomputed" in r.times else None,
                 queue_s=round(r.times["admitted"] - r.times["prepared"], 3),
                 gpu_s=round(r.times["finish"] - r.times["start"], 3),
                 residual=None if math.isnan(r.residual) else r.residual)
@@ -300,12 +298,13 @@ def main(argv=None):
     ap.add_argument("--chunk", type=int, default=50)
     ap.add_argument("--sample", type=float, default=0.05, help="fraction checked against FP64")
     ap.add_argument("--seed", type=int, default=3)
-    ap.add_argument("--host-precompute", action="store_true", help="RBF precompute on the host threads")
+    ap.add_argument("--gpu-precompute", action="store_true",
+                    help="RBF precompute on the GPU at admission (on the scheduler thread; default: host threads)")
     args = ap.parse_args(argv)
     reqs = request_stream(args.rate, args.duration, args.seed, args.nc_min, args.nc_max, args.t_end, args.res_tol,
                           args.max_steps)
     srv = Server(max_active=args.max_active, chunk=args.chunk, log=lambda m: print(m, flush=True),
-                 gpu_precompute=not args.host_precompute)
+                 gpu_precompute=args.gpu_precompute)
     done, wall = srv.run(reqs)
     tts = np.array([r.times["finish"] - r.arrival for r in done])
     cell_steps = float(sum(r.cells * r.steps for r in done))

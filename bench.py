@@ -4,9 +4,11 @@
 Workload (default): the 1000x1000 jittered unit square (+-20%, seed 1), each
 quad split into two triangles with probability 0.5 -> 1,500,605 cells;
 four-quadrant Riemann initial data; transmissive boundaries; AFAR switch
-with KFVS/Roe; all-FP64; CFL 0.5.  A step = CFL time-step reduction + one
-SSP-RK3 step (3 stages), time step kept on the device, replayed as a
-hipGraph; inputs resident in HBM.  Prints one JSON line.
+with KFVS/Roe; all-FP64; CFL 0.5.  The run is first advanced (untimed) to
+t = 0.15, the middle of the t = 0.3 run, where the waves cover much of the
+domain; then W warm-up steps and K timed steps.  A step = CFL time-step
+reduction + one SSP-RK3 step (3 stages), time step kept on the device,
+replayed as a hipGraph; inputs resident in HBM.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--case riemann|corner] [--precision fp64|mixed]
 """
@@ -24,24 +26,48 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBPS = 8000.0
 
 
-def mesh_stats(mesh):
-    S, Cc, _, cf = mesh.stencils()
+def mesh_stats(mesh, field=None, precision="fp64"):
+    """Mesh sizes for the byte model; with field (write_field(): primitive
+    state in caller order, ghosts, types) also the cells that take the
+    reconstruction's full path (T not constant over the compact stencil,
+    cudaLimiter.h:207-233): only those read their RBF coefficients and
+    neighbour-centre offsets.  The test runs in the reconstruction's type:
+    FP32-mixed rounds the primitive state to float first, as the device
+    state does."""
+    S, Cc, sten, cf = mesh.stencils()
     s = mesh.sizes()
-    nf = s["nf"]
+    nf, nc = s["nf"], s["nc"]
     internal = (np.abs(cf) - 1 < nf) & (cf != 0)
     n_int = internal.sum(axis=1)
-    return dict(nc=s["nc"], nf=nf, nb=s["nb"], sumS=int(S.sum()), sumC=int(Cc.sum()),
-                sum_int_coef=int((n_int * (S.astype(np.int64) + 1)).sum()), meanS=float(S.mean()),
-                meanC=float(Cc.mean()))
+    coef = n_int * (S.astype(np.int64) + 1)
+    full = np.ones(nc, dtype=bool)
+    if field is not None:
+        W, ghost, _ = field
+        Tm = np.empty(nc + s["nb"])
+        Tm[:nc] = W[mesh.permutation(), 3]  # mesh order
+        Tm[nc:] = ghost[:, 3]
+        cols = np.arange(sten.shape[1])[None, :] < Cc[:, None]
+        Ts = np.where(cols, Tm[np.where(cols, sten, 0)], Tm[:nc, None])
+        ft = np.float64 if precision == "fp64" else np.float32
+        Tm, Ts = Tm.astype(ft), Ts.astype(ft)
+        rng = np.maximum(Ts.max(axis=1), Tm[:nc]) - np.minimum(Ts.min(axis=1), Tm[:nc])
+        full = ~(np.abs(rng) < ft(1e-10))
+    return dict(nc=nc, nf=nf, nb=s["nb"], sumS=int(S.sum()), sumC=int(Cc.sum()),
+                sum_int_coef=int(coef.sum()), sum_int_coef_full=int(coef[full].sum()),
+                int_slots_full=int(n_int[full].sum()), full_cells=int(full.sum()),
+                meanS=float(S.mean()), meanC=float(Cc.mean()))
 
 
 def kernel_bytes(st, w=8):
     """Compulsory bytes per launch of each kernel (every array element touched
     once; neighbour re-reads counted as cache hits).  w = width of the
-    reconstruction/flux type (8 FP64, 4 FP32-mixed); state stays FP64."""
-    nc, nf, nb, sS, sC, sR = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"], st["sum_int_coef"]
+    reconstruction/flux type (8 FP64, 4 FP32-mixed); state stays FP64.  The
+    reconstruction reads the RBF coefficients and the neighbour-centre offsets
+    (slot_e) only for cells on the full path."""
+    nc, nf, nb, sS, sC = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"]
+    sR, sfull = st["sum_int_coef_full"], st["int_slots_full"]
     ns = 2 * nf  # internal face slots
-    recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 16 * ns + 16 * ns + 2 * w * sC + w * sR
+    recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 16 * ns + 16 * sfull + 2 * w * sC + w * sR
              + 8 * w * nc + 4 * w * nc + nc + 4 * w * ns + w * ns)
     bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + 16 * ns + w * ns + 4 * w * nc + 2 * w * nc + w * nc
            + nc + 4 * w * nc)
@@ -65,8 +91,9 @@ def survey_balg_per_cell_step(st, w=8):
 
 def measured_traffic(kernel, case, precision):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a copy
-    kernel's reading), or None when there is none."""
+    summary of this workload (the rows hbm_table prints: read requests by
+    size and WRITE_SIZE, each scaled by a copy kernel's reading), or None
+    when there is none."""
     path = os.path.join(ROOT, "profiles", "riemann1000_%s_hbm_traffic.txt" % precision)
     if case != "riemann" or not os.path.exists(path):
         return None
@@ -75,6 +102,62 @@ def measured_traffic(kernel, case, precision):
         if f and not line.startswith("#") and f[0] == kernel and len(f) == 5:
             return int(f[4])
     return None
+
+
+KERNEL_KEYS = [("k_reconstruct", "reconstruct"), ("k_bvd", "bvd"), ("k_flux<double, true>", "flux_dt"),
+               ("k_flux<float, true>", "flux_dt"), ("k_flux<", "flux"), ("k_dt_reduce", "dt_reduce"),
+               ("k_update", "update"), ("k_cfl", "cfl")]
+
+
+def _pmc_per_dispatch(csv_path):
+    """{dispatch id: (kernel name, {counter: value})} from a rocprofv3
+    counter_collection.csv."""
+    import csv
+    out = {}
+    for r in csv.DictReader(open(csv_path)):
+        d = out.setdefault(r["Dispatch_Id"], (r["Kernel_Name"], {}))
+        d[1][r["Counter_Name"]] = d[1].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return out
+
+
+def _read_bytes(c):
+    # TCC read requests by size; FETCH_SIZE counts a 128 B request as 64 B
+    return (32 * c.get("TCC_EA0_RDREQ_32B_sum", 0.0) + 64 * c.get("TCC_EA0_RDREQ_64B_sum", 0.0)
+            + 128 * c.get("TCC_EA0_RDREQ_128B_sum", 0.0))
+
+
+def hbm_table(reads_csv, writes_csv, copy_reads_csv, copy_writes_csv, copy_mib=1024):
+    """HBM bytes per launch of each solver kernel from two rocprofv3 --pmc
+    runs (TCC_EA0_RDREQ_{32B,64B,128B}_sum, then WRITE_SIZE), each scaled by
+    what the same counters read for a device-to-device copy of copy_mib MiB
+    (bench.py --copy-calibration).  Prints one row per kernel:
+    kernel dispatches read_bytes write_bytes traffic_bytes."""
+    size = copy_mib * (1 << 20)
+
+    def copies(path, fn):
+        v = [fn(c) for name, c in _pmc_per_dispatch(path).values() if "copyBuffer" in name]
+        return max(v) if v else None
+
+    rd = copies(copy_reads_csv, _read_bytes)
+    wr = copies(copy_writes_csv, lambda c: 1024.0 * c.get("WRITE_SIZE", 0.0))
+    fr, fw = size / rd, size / wr
+    print("# calibration: a %d MiB device-to-device copy reads %.0f B (32/64/128 B requests) and writes %.0f B "
+          "(WRITE_SIZE) -> factors read x%.4f, write x%.4f" % (copy_mib, rd, wr, fr, fw))
+    print("# kernel  dispatches  read_bytes  write_bytes  traffic_bytes")
+    acc = {}
+    for path, fn, k in [(reads_csv, _read_bytes, 0), (writes_csv, lambda c: 1024.0 * c.get("WRITE_SIZE", 0.0), 1)]:
+        for name, c in _pmc_per_dispatch(path).values():
+            key = next((v for pat, v in KERNEL_KEYS if pat in name), None)
+            if key is None:
+                continue
+            a = acc.setdefault(key, [[], []])
+            a[k].append(fn(c))
+    for key in dict.fromkeys(v for _, v in KERNEL_KEYS):
+        if key not in acc or not acc[key][0] or not acc[key][1]:
+            continue
+        r = float(np.mean(acc[key][0])) * fr
+        w = float(np.mean(acc[key][1])) * fw
+        print("%s %d %.0f %.0f %d" % (key, len(acc[key][0]), r, w, int(round(r + w))))
 
 
 def cpu_baseline(mesh, sc, W0, budget_s=12.0, max_steps=40):
@@ -94,8 +177,8 @@ def cpu_baseline(mesh, sc, W0, budget_s=12.0, max_steps=40):
     el = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return dict(value=mesh.nc * n / el / 1e6, unit="Mcell-RK3-steps/s", cores=cores, kind="port",
-                sample="%d RK3 steps (with CFL) of the same %d-cell mesh, FP64 oracle, %d OpenMP threads, %.1f s"
-                       % (n, mesh.nc, cores, el))
+                sample="%d RK3 steps (with CFL) of the same %d-cell mesh from the same state, FP64 oracle, "
+                       "%d OpenMP threads, %.1f s" % (n, mesh.nc, cores, el))
 
 
 def main():
@@ -107,16 +190,33 @@ def main():
                     help="batch: 256 corner flows (5k-200k cells, seed 2), FP32-mixed, autotuned per size bucket")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--tile", type=int, default=64)
-    ap.add_argument("--recon-lanes", type=int, default=0, choices=[0, 1, 4],
-                    help="reconstruction threads per cell (0: 4 when every cell has at most four faces)")
+    ap.add_argument("--t-start", type=float, default=None,
+                    help="simulated time reached (untimed) before the warm-up; default 0.15 for the Riemann case "
+                         "(mid-run of t = 0.3, when the waves cover much of the domain), 0 otherwise")
+    ap.add_argument("--recon-lanes", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="reconstruction kernel: 1 one thread per cell, 4 four per cell, 2 four per cell with "
+                         "the face work compacted; 0 automatic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--faithful-boundary", action="store_true",
                     help="riemann case without the boundaryFirstOrder extension (loses positivity)")
     ap.add_argument("--batch-size", type=int, default=256, help="meshes of the batch case")
     ap.add_argument("--batch-fp64", action="store_true", help="batch case in FP64 (default FP32-mixed)")
+    ap.add_argument("--save-state", metavar="NPZ", default=None,
+                    help="after reaching t-start (untimed), save the conserved state and time to NPZ and exit")
+    ap.add_argument("--restart", metavar="NPZ", default=None,
+                    help="start from a state saved by --save-state instead of advancing to t-start "
+                         "(keeps profiled runs short)")
+    ap.add_argument("--precompute-timing", action="store_true",
+                    help="only time the RBF precompute, host vs GPU, on the Riemann meshes (250^2 and 1000^2 nodes)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (calibrates HBM byte counters) and exit")
+    ap.add_argument("--hbm-table", nargs=4, default=None,
+                    metavar=("READS_CSV", "WRITES_CSV", "COPY_READS_CSV", "COPY_WRITES_CSV"),
+                    help="only print HBM bytes per launch per kernel from rocprofv3 --pmc counter_collection.csv "
+                         "files (no GPU needed)")
     args = ap.parse_args()
+    if args.hbm_table:
+        return hbm_table(*args.hbm_table, copy_mib=args.copy_calibration or 1024)
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding); "
               "use --gpus 1", file=sys.stderr)
@@ -136,6 +236,8 @@ def main():
     from afar_cudaeuler2dmixo3rk3solver_amd import cases
     if args.case == "batch":
         return run_batch(args)
+    if args.precompute_timing:
+        return run_precompute_timing()
 
     t0 = time.perf_counter()
     if args.case == "riemann":
@@ -153,9 +255,30 @@ def main():
     mesh.precompute(sc)
     t_pre = time.perf_counter() - t0
     print("bench: %s, nc=%d, precompute %.1f s" % (args.case, mesh.nc, t_pre), file=sys.stderr, flush=True)
-    st = mesh_stats(mesh)
     gpu = A.BasicCUDASolver(mesh, sc, W0)
     dev_name = A.device_info()[0]
+    t_start = args.t_start if args.t_start is not None else (0.15 if args.case == "riemann" else 0.0)
+    pre_steps = 0
+    if args.restart:
+        z = np.load(args.restart, allow_pickle=False)
+        if int(z["cells"]) != mesh.nc or str(z["precision"]) != args.precision:
+            raise SystemExit("bench: %s holds another workload" % args.restart)
+        gpu.set_conserved(z["U"])
+        gpu.set_time(float(z["t"]))
+        t_start, pre_steps = float(z["t"]), int(z["steps"])
+        print("bench: restarted from %s at t = %.4f" % (args.restart, t_start), file=sys.stderr, flush=True)
+    elif t_start > 0:
+        gpu.run_until(t_start, chunk=100)
+        pre_steps = gpu.device_steps()
+        gpu.set_time(gpu.device_time())  # clock keeps running, no end time
+        print("bench: advanced to t = %.4f in %d steps (untimed)" % (gpu.device_time(), pre_steps),
+              file=sys.stderr, flush=True)
+    if args.save_state:
+        np.savez(args.save_state, U=gpu.conserved(), t=gpu.device_time(), steps=pre_steps, cells=mesh.nc,
+                 precision=args.precision)
+        print("bench: state at t = %.4f saved to %s" % (gpu.device_time(), args.save_state), file=sys.stderr)
+        return
+    W_start = gpu.write_field()[0]  # the CPU baseline starts from the same state
 
     gpu.evolve_auto(args.warmup)
     gpu.synchronize()
@@ -168,8 +291,11 @@ def main():
     print("bench: %d steps in %.3f s (events) / %.3f s (wall)" % (args.steps, seconds, wall), file=sys.stderr,
           flush=True)
 
-    W_end, _, _ = gpu.write_field()
+    field = gpu.write_field()
+    W_end = field[0]
     finite = bool(np.isfinite(W_end).all() and (W_end[:, 0] > 0).all() and (W_end[:, 3] > 0).all())
+    st = mesh_stats(mesh, field, args.precision)
+    t_end = gpu.device_time()
     prof = gpu.profile(reps=20)  # back-to-back launches per kernel; after the timed run
     w = 8 if args.precision == "fp64" else 4
     kb = kernel_bytes(st, w)
@@ -187,7 +313,7 @@ def main():
                     alg_bytes_per_launch=int(d["alg_bytes"]), avg_launch_ms=round(d["avg_ms"], 5))
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(mesh, sc, W0)
+        cpu = cpu_baseline(mesh, sc, W_start)
     out = {
         "metric": "Mcell-RK3-steps/s", "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_events / args.steps, 4),
@@ -198,7 +324,9 @@ def main():
                    "mean_compact": round(st["meanC"], 3), "precision": args.precision, "cfl": 0.5,
                    "tile": args.tile, "recon_lanes": args.recon_lanes or "auto", "device": dev_name.strip(),
                    "precompute_s": round(t_pre, 1),
-                   "boundary_first_order": bfo, "state_positive_after_timed_steps": finite},
+                   "boundary_first_order": bfo, "state_positive_after_timed_steps": finite,
+                   "t_start": t_start, "untimed_steps_to_t_start": pre_steps, "t_after_timed_steps": round(t_end, 5),
+                   "full_path_cells": st["full_cells"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
@@ -209,6 +337,24 @@ def main():
                            sum(kb[k] * kernels[k]["count_per_step"] for k in kb) / mesh.nc, 1)},
     }
     print(json.dumps(out))
+
+
+def run_precompute_timing():
+    """Host precompute (all host threads) against the GPU precompute, and
+    whether the GPU coefficients equal the host's bit for bit."""
+    import afar_cudaeuler2dmixo3rk3solver_amd as A
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    sc = A.schemes(cOrder=7, fOrder=2)
+    for n in (250, 1000):
+        a, _ = cases.riemann_case(n=n, seed=1)
+        b, _ = cases.riemann_case(n=n, seed=1)
+        t0 = time.perf_counter()
+        a.precompute(sc)
+        t1 = time.perf_counter()
+        b.precompute(sc, gpu=True)
+        t2 = time.perf_counter()
+        print(json.dumps(dict(n=n, cells=a.nc, host_s=round(t1 - t0, 3), gpu_s=round(t2 - t1, 3),
+                              equal=bool(np.array_equal(a.rbf_basis()[0], b.rbf_basis()[0])))), flush=True)
 
 
 def run_batch(args):

@@ -61,8 +61,10 @@ typedef struct afar_schemes {
     int32_t boundaryFirstOrder; /* extension, 0 = reference behaviour: cells with a boundary face
                                    use first-order face states (the reference does this only for
                                    cells with two boundary faces, cudaLimiter.h:419-429) */
-    int32_t reconLanes;  /* threads per cell of the reconstruction kernel: 0 automatic (4 when
-                            every cell has at most four faces), 1, or 4; same results */
+    int32_t reconLanes;  /* reconstruction kernel, same results: 0 automatic, 1 one thread per
+                            cell, 4 four threads per cell, 2 four threads per cell with the
+                            tile's face work compacted (2 and 4: cells of at most four faces;
+                            2: tiles of at most 128 cells; automatic picks 4, else 1) */
 } afar_schemes;
 
 const char* afar_last_error(void);

@@ -47,3 +47,14 @@ def test_corner_residual_stop(afar):
     mesh, W0 = build_case(r, srv.sc)
     srv.solve(r, mesh, W0, time.perf_counter())
     assert r.steps == 10  # the first check already meets the tolerance
+
+
+def test_served_with_gpu_precompute_equals_host(afar):
+    """Requests precomputed on the GPU at admission give the results of the
+    host-precomputed run (the GPU precompute is bitwise the host's)."""
+    from afar_cudaeuler2dmixo3rk3solver_amd.serve import Server
+    print("case: 4 requests, GPU precompute vs host precompute (mixed)", flush=True)
+    a, _ = Server(precision="mixed", max_active=4, chunk=20, gpu_precompute=True).run(make_requests(), realtime=False)
+    b, _ = Server(precision="mixed", max_active=4, chunk=20, gpu_precompute=False).run(make_requests(), realtime=False)
+    for x, y in zip(a, b):
+        assert x.steps == y.steps and np.array_equal(x.result, y.result), x.rid
