@@ -26,6 +26,8 @@ template <typename R>
 struct DevPtrs {
     int nc, nf, nb, maxS, maxC, ntiles;
     int lds_sten_off;          // byte offset of the stencil ids in the reconstruction kernels' LDS
+    int lds_m_off;             // byte offset of the tile's least-squares weights (four-lane kernel)
+    int lds_bvd_sten_off;      // byte offset of the stencil ids in the BVD kernel's LDS
     int boundaryFirstOrder;    // extension: first-order faces in cells with a boundary face
     typename VT<R>::v4* W;     // nc+nb primitive p, Ux, Uy, T
     double4* U;                // nc conserved rho, rhoUx, rhoUy, rhoE

@@ -298,87 +298,142 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
 // reductions over the group (the limiter is a minimum, the smoothness a
 // maximum, the trouble test that of the last internal face), so the values
 // equal the one-lane-per-cell kernel's bit for bit.
-// Minimum waves per SIMD the register allocator must allow for the four-lane
-// kernel (0: no constraint); a build-time tuning knob.
-#ifndef AFAR_RECONQ_WPE
-#define AFAR_RECONQ_WPE 0
+// Waves per SIMD the register allocator must allow for the four-lane kernel,
+// per working type (build-time tuning knobs; 1 = no constraint).  5 (FP64, 96
+// VGPRs) and 8 (FP32, 64 VGPRs) fit without spills and were measured faster
+// than the unconstrained 4 and 7 waves.
+#ifndef AFAR_RECONQ_WPE_D
+#define AFAR_RECONQ_WPE_D 5
 #endif
-#if AFAR_RECONQ_WPE > 0
-#define AFAR_RECONQ_ATTR __attribute__((amdgpu_waves_per_eu(AFAR_RECONQ_WPE, 8)))
-#else
-#define AFAR_RECONQ_ATTR
+#ifndef AFAR_RECONQ_WPE_F
+#define AFAR_RECONQ_WPE_F 8
 #endif
+template <typename R> struct ReconQWaves {
+    static constexpr int value = sizeof(R) == 8 ? AFAR_RECONQ_WPE_D : AFAR_RECONQ_WPE_F;
+};
+#define AFAR_RECONQ_ATTR __attribute__((amdgpu_waves_per_eu(ReconQWaves<R>::value, 8)))
 
 template <typename R, int TILE>
 __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(DevPtrs<R> p) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
+    constexpr int NT = 4 * TILE;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     v4* sW = reinterpret_cast<v4*>(smem_raw);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= p.ntiles) return;
     const int t = threadIdx.x;
-    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
-    uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
-    if (t < TILE && tile * TILE + t < p.nc) sW[t] = p.W[tile * TILE + t];
-    for (int i = t; i < hn; i += 4 * TILE) sW[TILE + i] = p.W[p.halo[hb + i]];
-    stage_stencil(sSt, p.sten, tile, TILE, p.nc, p.maxS, t, 4 * TILE);
-    __syncthreads();
     const int lc = t >> 2, k = t & 3;
     const int c = tile * TILE + lc;
-    if (c >= p.nc) return;  // the four lanes of a cell leave together
-    const int S = p.scnt[c], C = p.ccnt[c];
-    const int maxS = p.maxS, maxC = p.maxC, KR = maxC * (maxS + 1);
+    const int maxS = p.maxS, maxC = p.maxC, KR = maxC * (maxS + 1), nf = p.nf;
+    // Every load that needs no LDS is issued before the staging barrier, in
+    // two dependent rounds: (1) the cell's counts, the tile's own states,
+    // halo ids and stencil block; (2) lane k's least-squares weight and slot
+    // data, and the halo states.  Their latencies overlap.
+    const bool inb = c < p.nc;
+    int S = 0, C = 0, sf = 0;
+    if (inb) {
+        S = p.scnt[c];
+        C = p.ccnt[c];
+    }
+    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
+    const int wpt = TILE >> 6;  // stencil block: (waves of the tile) * maxS * 64 ids
+    const int nw = min(wpt, ((p.nc + 63) >> 6) - tile * wpt);
+    const int n16 = nw * maxS * 8;
+    const uint4* stSrc = reinterpret_cast<const uint4*>(p.sten + (size_t)tile * wpt * maxS * 64);
+    const bool ownW = t < TILE && tile * TILE + t < p.nc;
+    v4 wOwn;
+    if (ownW) wOwn = p.W[tile * TILE + t];
+    uint4 st0;
+    if (t < n16) st0 = stSrc[t];
+    int h0 = 0;
+    if (t < hn) h0 = p.halo[hb + t];
+    v2 Mk = v2{(R)0, (R)0};
+    double2 d = {0.0, 0.0};
+    if (inb && k < C) {
+        Mk = p.lsq[sell(c, k, maxC)];
+        sf = p.slot_face[sell(c, k, maxC)];
+        d = p.slot_d[sell(c, k, maxC)];
+    }
+    v4 wHalo;
+    if (t < hn) wHalo = p.W[h0];
+    uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
+    v2* sM = reinterpret_cast<v2*>(smem_raw + p.lds_m_off);  // 4 weights per cell
+    uint4* stDst = reinterpret_cast<uint4*>(sSt);
+    if (ownW) sW[t] = wOwn;
+    if (t < hn) sW[TILE + t] = wHalo;
+    if (t < n16) stDst[t] = st0;
+    sM[t] = Mk;
+    for (int i = t + NT; i < hn; i += NT) sW[TILE + i] = p.W[p.halo[hb + i]];  // halos beyond NT cells
+    for (int i = t + NT; i < n16; i += NT) stDst[i] = stSrc[i];
+    __syncthreads();
+    if (!inb) return;  // the four lanes of a cell leave together
     const v4 wc = sW[lc];
 
-    R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
+    // the temperature range over the compact stencil selects the branch
+    R TMax = wc.w, TMin = wc.w;
     for (int i = 0; i < C; ++i) {
-        const v2 M = p.lsq[sell(c, i, maxC)];
-        const v4 wj = sW[sSt[sell(lc, i, maxS)]];
-        R dphi = wj.x - wc.x;
-        gpx += M.x * dphi;
-        gpy += M.y * dphi;
-        dphi = wj.y - wc.y;
-        guxx += M.x * dphi;
-        guyx += M.y * dphi;
-        dphi = wj.z - wc.z;
-        guxy += M.x * dphi;
-        guyy += M.y * dphi;
-        dphi = wj.w - wc.w;
-        gTx += M.x * dphi;
-        gTy += M.y * dphi;
+        const R Tj = sW[sSt[sell(lc, i, maxS)]].w;
+        TMax = fmax(Tj, TMax);
+        TMin = fmin(Tj, TMin);
     }
-    if (k == 0) p.gp[c] = v2{gpx, gpy};
-    if (k == 1) p.gu[c] = v2{guxx, guyx};
-    if (k == 2) p.gv[c] = v2{guxy, guyy};
-    if (k == 3) p.gT[c] = v2{gTx, gTy};
-
-    R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z, TMax = wc.w, TMin = wc.w;
-    for (int i = 0; i < C; ++i) {
-        const v4 wj = sW[sSt[sell(lc, i, maxS)]];
-        pMax = fmax(wj.x, pMax);
-        pMin = fmin(wj.x, pMin);
-        uxMax = fmax(wj.y, uxMax);
-        uxMin = fmin(wj.y, uxMin);
-        uyMax = fmax(wj.z, uyMax);
-        uyMin = fmin(wj.z, uyMin);
-        TMax = fmax(wj.w, TMax);
-        TMin = fmin(wj.w, TMin);
-    }
-
-    const int nf = p.nf;
-    const int sf = k < C ? p.slot_face[sell(c, k, maxC)] : 0;
     const int isNei = sf < 0;
     const int f = (sf < 0 ? -sf : sf) - 1;
     const bool live = k < C && f < nf;
-    if (fabs(TMax - TMin) < K<R>::VSMALL) {
+    const bool flat = fabs(TMax - TMin) < K<R>::VSMALL;
+
+    // Full path: every RBF coefficient of the face and its centre offset are
+    // requested at once (one memory round trip).
+    R cf[kMaxSten + 1];
+    double2 e = {0.0, 0.0};
+    const R* a = p.rbf + sell(c, k * (maxS + 1), KR);  // row stride 64
+    if (!flat && live) {
+        e = p.slot_e[sell(c, k, maxC)];
+#pragma unroll
+        for (int i = 0; i <= kMaxSten; ++i)
+            if (i <= S) cf[i] = a[(size_t)i * 64];
+    }
+    const R dx = (R)d.x, dy = (R)d.y;
+    R deltaR = 0, fraction = 0;
+    if (!flat && live) {  // the offset arrives first; two values instead of four stay live
+        deltaR = (R)afar_hypot(e.x, e.y);
+        fraction = sqrt(dx * dx + dy * dy) / deltaR;
+    }
+
+    // least-squares gradient, from the staged weights (same order as the
+    // one-lane kernel); the full path forms it after the RBF sums, so that
+    // fewer values are live while the coefficients are in registers
+    R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
+    auto gradient = [&]() {
+        for (int i = 0; i < C; ++i) {
+            const v2 M = sM[4 * lc + i];
+            const v4 wj = sW[sSt[sell(lc, i, maxS)]];
+            R dphi = wj.x - wc.x;
+            gpx += M.x * dphi;
+            gpy += M.y * dphi;
+            dphi = wj.y - wc.y;
+            guxx += M.x * dphi;
+            guyx += M.y * dphi;
+            dphi = wj.z - wc.z;
+            guxy += M.x * dphi;
+            guyy += M.y * dphi;
+            dphi = wj.w - wc.w;
+            gTx += M.x * dphi;
+            gTy += M.y * dphi;
+        }
+        if (k == 0) p.gp[c] = v2{gpx, gpy};
+        if (k == 1) p.gu[c] = v2{guxx, guyx};
+        if (k == 2) p.gv[c] = v2{guxy, guyy};
+        if (k == 3) p.gT[c] = v2{gTx, gTy};
+    };
+
+    if (flat) {
+        gradient();
         if (k == 0) {
             p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
             p.smooth[c] = 0;
         }
         if (live) {
-            const double2 d = p.slot_d[sell(c, k, maxC)];
-            const R dx = (R)d.x, dy = (R)d.y;
             v4 o;
             o.x = wc.x + gpx * dx + gpy * dy;
             o.y = wc.y + guxx * dx + guyx * dy;
@@ -390,28 +445,43 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         return;
     }
 
+    R pf = 0, uxf = 0, uyf = 0, Tf = 0;
+    if (live) {
+        R coef = cf[0];
+        pf = coef * wc.x;
+        uxf = coef * wc.y;
+        uyf = coef * wc.z;
+        Tf = coef * wc.w;
+#pragma unroll
+        for (int ci = 0; ci < kMaxSten; ++ci) {
+            if (ci < S) {
+                const v4 wj = sW[sSt[sell(lc, ci, maxS)]];
+                coef = cf[ci + 1];
+                pf += coef * wj.x;
+                uxf += coef * wj.y;
+                uyf += coef * wj.z;
+                Tf += coef * wj.w;
+            }
+        }
+    }
+    gradient();
+    R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z;
+    for (int i = 0; i < C; ++i) {
+        const v4 wj = sW[sSt[sell(lc, i, maxS)]];
+        pMax = fmax(wj.x, pMax);
+        pMin = fmin(wj.x, pMin);
+        uxMax = fmax(wj.y, uxMax);
+        uxMin = fmin(wj.y, uxMin);
+        uyMax = fmax(wj.z, uyMax);
+        uyMin = fmin(wj.z, uyMin);
+    }
+
     R limP = 2, limUx = 2, limUy = 2, limT = 2;
     R smoothness = 0;
     int faceNotTrouble = 1;
     if (live) {
         const R rP = fabs(pMax - pMin), rUx = fabs(uxMax - uxMin), rUy = fabs(uyMax - uyMin), rT = fabs(TMax - TMin);
-        const double2 d = p.slot_d[sell(c, k, maxC)];
-        const R dx = (R)d.x, dy = (R)d.y;
-        const double2 e = p.slot_e[sell(c, k, maxC)];
-        const R deltaR = (R)afar_hypot(e.x, e.y);
-        const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
         const int nbl = sSt[sell(lc, k, maxS)];
-        const R* a = p.rbf + sell(c, k * (maxS + 1), KR);
-        R coef = a[0];
-        R pf = coef * wc.x, uxf = coef * wc.y, uyf = coef * wc.z, Tf = coef * wc.w;
-        for (int ci = 0; ci < S; ++ci) {
-            const v4 wj = sW[sSt[sell(lc, ci, maxS)]];
-            coef = a[(size_t)(ci + 1) * 64];
-            pf += coef * wj.x;
-            uxf += coef * wj.y;
-            uyf += coef * wj.z;
-            Tf += coef * wj.w;
-        }
         v4 o;
         R phif;
         limiter_process(limP, phif, wc.x, pMax, pMin, gpx, gpy, dx, dy);
@@ -666,54 +736,114 @@ template <typename R, int TILE>
 __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
+    constexpr int PF = 4;  // face slots whose data is requested before the barrier
     extern __shared__ __align__(16) unsigned char smem_raw[];
     int8_t* sS = reinterpret_cast<int8_t*>(smem_raw);
+    uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_bvd_sten_off);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     if (tile >= p.ntiles) return;
     const int tid = threadIdx.x, c = tile * TILE + tid;
-    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
-    if (c < p.nc) sS[tid] = p.smooth[c];
-    for (int i = tid; i < hn; i += TILE) sS[TILE + i] = p.smooth[p.halo[hb + i]];
-    __syncthreads();
-    if (c >= p.nc) return;
-    const int S = p.scnt[c], C = p.ccnt[c];
     const int maxS = p.maxS, maxC = p.maxC, nf = p.nf;
-    int8_t shockType = sS[tid];
-    int8_t shockTypeExt = sS[tid];
+    const bool inb = c < p.nc;
+    // Two rounds of independent loads before the barrier: (1) the cell's
+    // counts, flag, limiter, temperature and its gradient, the first slots'
+    // faces, halo ids and the stencil block; (2) halo flags and the slots'
+    // face temperatures and offsets.  Indices are clamped rather than
+    // branched on, and values held in plain scalars, so that no load is
+    // waited for before its round is complete.
+    const int cc = inb ? c : p.nc - 1;
+    const int S = p.scnt[cc], C = inb ? (int)p.ccnt[cc] : 0;
+    const int8_t own = p.smooth[cc];
+    const v4 limc = p.lim[cc];
+    const v2 gT = p.gT[cc];
+    const R Tc = p.W[cc].w;
+    int sfs[PF];
+#pragma unroll
+    for (int fi = 0; fi < PF; ++fi) sfs[fi] = fi < maxC ? p.slot_face[sell(cc, fi, maxC)] : 0;
+    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
+    const int wpt = TILE >> 6;
+    const int nw = min(wpt, ((p.nc + 63) >> 6) - tile * wpt);
+    const int n16 = nw * maxS * 8;
+    const uint4* stSrc = reinterpret_cast<const uint4*>(p.sten + (size_t)tile * wpt * maxS * 64);
+    const uint4 st0 = stSrc[min(tid, n16 - 1)];
+    const uint4 st1 = stSrc[min(tid + TILE, n16 - 1)];
+    const uint4 st2 = stSrc[min(tid + 2 * TILE, n16 - 1)];
+    const int hmax = hn > 0 ? hn - 1 : 0;  // a tile without halo reads its first id harmlessly
+    const unsigned hid0 = (unsigned)p.halo[hb + min(tid, hmax)];
+    const unsigned hid1 = (unsigned)p.halo[hb + min(tid + TILE, hmax)];
+    const int8_t hs0 = p.smooth[hid0];
+    const int8_t hs1 = p.smooth[hid1];
+    R tfo[PF], tfn[PF];
+    double dxs[PF], dys[PF];
+#pragma unroll
+    for (int fi = 0; fi < PF; ++fi) {
+        const int sf = sfs[fi];
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        const bool in = fi < C && f < nf;
+        const int fs = in ? f : 0;
+        tfo[fi] = p.fvT[2 * fs];
+        tfn[fi] = p.fvT[2 * fs + 1];
+        // a slot without an internal face reads slot 0's offset (a line read anyway)
+        const double2 d = p.slot_d[sell(cc, in ? fi : 0, maxC)];
+        dxs[fi] = d.x;
+        dys[fi] = d.y;
+    }
+    if (inb) sS[tid] = own;
+    uint4* stDst = reinterpret_cast<uint4*>(sSt);
+    if (tid < hn) sS[TILE + tid] = hs0;
+    if (tid + TILE < hn) sS[2 * TILE + tid] = hs1;
+    if (tid < n16) stDst[tid] = st0;
+    if (tid + TILE < n16) stDst[tid + TILE] = st1;
+    if (tid + 2 * TILE < n16) stDst[tid + 2 * TILE] = st2;
+    for (int i = tid + 2 * TILE; i < hn; i += TILE) sS[TILE + i] = p.smooth[p.halo[hb + i]];
+    for (int i = tid + 3 * TILE; i < n16; i += TILE) stDst[i] = stSrc[i];
+    __syncthreads();
+    if (!inb) return;
+    int8_t shockType = own;
+    int8_t shockTypeExt = own;
     for (int ci = 0; ci < C; ++ci) {
-        shockType += sS[p.sten[sell(c, ci, maxS)]];
+        shockType += sS[sSt[sell(tid, ci, maxS)]];
         shockTypeExt = shockType;
     }
-    for (int ci = C; ci < S; ++ci) shockTypeExt += sS[p.sten[sell(c, ci, maxS)]];
+    for (int ci = C; ci < S; ++ci) shockTypeExt += sS[sSt[sell(tid, ci, maxS)]];
 
-    const v4 limc = p.lim[c];
-    const v2 gT = p.gT[c];
-    const R Tc = p.W[c].w;
+    // Every face is visited; the reference leaves the loop at a cell's second
+    // boundary face with bvd = 1, the largest value, so the result is the same.
     int bvd = 0, bfaces = 0;
-    for (int fi = 0; fi < C; ++fi) {
-        const int sf = p.slot_face[sell(c, fi, maxC)];
+    const R lim = to_bj(limc.w);
+    auto face = [&](int sf, R TfOwn, R TfNei, double ddx, double ddy) {
         const int isNei = sf < 0;
-        const int f = (sf < 0 ? -sf : sf) - 1;
-        if (f >= nf) {
-            bfaces += 1;
-            if (bfaces >= 2) {  // corner cells use the second-order reconstruction
-                bvd = 1;
-                break;
-            }
-            continue;
-        }
-        const double2 d = p.slot_d[sell(c, fi, maxC)];
-        const R dx = (R)d.x, dy = (R)d.y;
-        const R TfOwn = p.fvT[2 * f], TfNei = p.fvT[2 * f + 1];
+        const R dx = (R)ddx, dy = (R)ddy;
         const R Tf_cur = (R)isNei * TfNei + (R)(1 - isNei) * TfOwn;
         const R Tf_nei = (R)(1 - isNei) * TfNei + (R)isNei * TfOwn;
-        const R lim = to_bj(limc.w);
         const R Tf_cur_TVD = Tc + lim * (gT.x * dx + gT.y * dy);
         const int typeval = fabs(Tf_cur_TVD - Tf_nei) < K<R>::BVD_FACTOR * fabs(Tf_cur - Tf_nei);
         const int isNotFlat = fabs(Tf_cur - Tf_nei) > K<R>::VSMALL;
         const int isLimited = (lim < K<R>::VALID_LIM_FACTOR) + (shockTypeExt > 0);
         bvd = max(bvd, isNotFlat * isLimited * typeval);
+    };
+#pragma unroll
+    for (int fi = 0; fi < PF; ++fi) {
+        if (fi < C) {
+            const int sf = sfs[fi];
+            const int f = (sf < 0 ? -sf : sf) - 1;
+            if (f >= nf)
+                bfaces += 1;
+            else
+                face(sf, tfo[fi], tfn[fi], dxs[fi], dys[fi]);
+        }
     }
+    for (int fi = PF; fi < C; ++fi) {  // cells with more than PF faces
+        const int sf = p.slot_face[sell(c, fi, maxC)];
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        if (f >= nf) {
+            bfaces += 1;
+        } else {
+            const double2 d = p.slot_d[sell(c, fi, maxC)];
+            face(sf, p.fvT[2 * f], p.fvT[2 * f + 1], d.x, d.y);
+        }
+    }
+    if (bfaces >= 2) bvd = 1;  // corner cells use the second-order reconstruction
     int finalType = (shockType > 0) * (shockTypeExt > 1) * (-2) + (bvd > 0);
     if (p.boundaryFirstOrder && bfaces > 0) {
         // extension: a cell with a boundary face takes the first-order state

@@ -58,6 +58,7 @@ class SolverImpl final : public DeviceSolver {
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         // LDS of the reconstruction: tile + halo states, then the tile's stencil ids
         lds_recon_ = (size_t)(L_.tile + L_.maxHalo) * sizeof(v4) + (size_t)L_.tile * L_.maxS * sizeof(uint16_t);
+        lds_m_off_ = (lds_recon_ + 15) & ~(size_t)15;
         if ((cfg.reconLanes == 4 || cfg.reconLanes == 2) && L_.maxC > 4)
             throw Error(1, "reconLanes 2 and 4 need cells with at most four faces");
         if (cfg.reconLanes == 2 && L_.tile > 128) throw Error(1, "reconLanes 2 needs a tile of at most 128 cells");
@@ -65,7 +66,11 @@ class SolverImpl final : public DeviceSolver {
             lanes_ = cfg.reconLanes;
         else
             lanes_ = L_.maxC > 4 ? 1 : 4;  // four lanes measured fastest at 64 and 128-cell tiles
-        lds_bvd_ = (size_t)(L_.tile + L_.maxHalo);
+        // the four-lane kernel also stages the tile's least-squares weights
+        if (lanes_ == 4) lds_recon_ = lds_m_off_ + (size_t)L_.tile * 4 * sizeof(v2);
+        // BVD: the tile's and halo's smoothness flags, then the stencil ids
+        lds_bvd_sten_off_ = ((size_t)(L_.tile + L_.maxHalo) + 15) & ~(size_t)15;
+        lds_bvd_ = lds_bvd_sten_off_ + (size_t)L_.tile * L_.maxS * sizeof(uint16_t);
         int dev = 0;
         check_hip(hipGetDevice(&dev), "hipGetDevice");
         hipDeviceProp_t prop;
@@ -425,6 +430,8 @@ class SolverImpl final : public DeviceSolver {
         P_.maxC = L_.maxC;
         P_.ntiles = L_.ntiles;
         P_.lds_sten_off = (int)((size_t)(L_.tile + L_.maxHalo) * sizeof(v4));
+        P_.lds_m_off = (int)lds_m_off_;
+        P_.lds_bvd_sten_off = (int)lds_bvd_sten_off_;
         P_.boundaryFirstOrder = cfg_.boundaryFirstOrder;
         m_W.alloc(sizeof(v4) * (nc + nb));
         P_.W = m_W.as<v4>();
@@ -460,7 +467,13 @@ class SolverImpl final : public DeviceSolver {
             P_.rbf = put(m_rbf, rbf.data(), rbf.size());
         }
         P_.halo_ptr = put(m_halo_ptr, L_.halo_ptr.data(), L_.halo_ptr.size());
-        P_.halo = put(m_halo, L_.halo.data(), L_.halo.size());
+        {
+            // one trailing id (cell 0) so that a tile without halo may read
+            // its first id unconditionally
+            std::vector<int32_t> halo(L_.halo.begin(), L_.halo.end());
+            halo.push_back(0);
+            P_.halo = put(m_halo, halo.data(), halo.size());
+        }
         P_.fcell = reinterpret_cast<const int2*>(put(m_fcell, L_.fcell.data(), 2 * nf));
         P_.fgeo = reinterpret_cast<const double4*>(put(m_fgeo, L_.fgeo.data(), 8 * nf));
         m_fv.alloc(sizeof(v4) * 2 * nf);
@@ -522,7 +535,7 @@ class SolverImpl final : public DeviceSolver {
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
-    size_t lds_recon_ = 0, lds_bvd_ = 0;
+    size_t lds_recon_ = 0, lds_bvd_ = 0, lds_m_off_ = 0, lds_bvd_sten_off_ = 0;
     int lanes_ = 1;  // threads per cell of the reconstruction kernel
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fv, m_flux, m_bcell,
