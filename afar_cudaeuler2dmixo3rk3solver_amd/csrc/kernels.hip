@@ -291,11 +291,32 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
     p.smooth[c] = smoothness > (R)1;
 }
 
+// Moves within groups of four consecutive lanes (DPP quad permutations, full
+// rate VALU); every lane of the group must be active.
+template <int CTRL> __device__ __forceinline__ int dpp_move(int x) {
+    return __builtin_amdgcn_mov_dpp(x, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL> __device__ __forceinline__ float dpp_move(float x) {
+    return __int_as_float(dpp_move<CTRL>(__float_as_int(x)));
+}
+template <int CTRL> __device__ __forceinline__ double dpp_move(double x) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)dpp_move<CTRL>((int)(unsigned)u);
+    const unsigned hi = (unsigned)dpp_move<CTRL>((int)(unsigned)(u >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// the value of lane J of this lane's group
+template <int J, typename T> __device__ __forceinline__ T quad_bcast(T x) { return dpp_move<J * 0x55>(x); }
+// the value of lane (this lane xor X) of the group, X = 1 or 2
+template <int X, typename T> __device__ __forceinline__ T quad_xor(T x) {
+    return dpp_move<X == 1 ? 0xB1 : 0x4E>(x);  // quad_perm [1,0,3,2] and [2,3,0,1]
+}
+
 // K5 with four lanes per cell, for meshes whose cells have at most four
 // faces (triangles and quads): lane k of a cell's group handles face slot k
-// (its RBF sum and limiter candidates); the gradient and the compact min/max
-// are formed by every lane of the group.  The per-cell results are exact
-// reductions over the group (the limiter is a minimum, the smoothness a
+// (its RBF sum and limiter candidates) and variable k of the gradient and the
+// compact min/max, which the group then shares.  The per-cell results are
+// exact reductions over the group (the limiter is a minimum, the smoothness a
 // maximum, the trouble test that of the last internal face), so the values
 // equal the one-lane-per-cell kernel's bit for bit.
 // Waves per SIMD the register allocator must allow for the four-lane kernel,
@@ -368,15 +389,37 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     for (int i = t + NT; i < n16; i += NT) stDst[i] = stSrc[i];
     __syncthreads();
     if (!inb) return;  // the four lanes of a cell leave together
-    const v4 wc = sW[lc];
+    const R* sWr = reinterpret_cast<const R*>(sW);
 
     // the temperature range over the compact stencil selects the branch
-    R TMax = wc.w, TMin = wc.w;
+    R TMax = sWr[4 * lc + 3], TMin = TMax;
     for (int i = 0; i < C; ++i) {
-        const R Tj = sW[sSt[sell(lc, i, maxS)]].w;
+        const R Tj = sWr[4 * sSt[sell(lc, i, maxS)] + 3];
         TMax = fmax(Tj, TMax);
         TMin = fmin(Tj, TMin);
     }
+    // Lane k forms variable k's least-squares gradient and compact-stencil
+    // extrema (the reference's per-variable sums, in the same order; the
+    // extrema are exact) and stores the gradient; the group then shares them.
+    // The full path does this after its RBF sums, so that fewer values are
+    // live while the coefficients are in registers.
+    R gxk = 0, gyk = 0, mxk = 0, mnk = 0;
+    auto gradient = [&]() {
+        const R wck = sWr[4 * lc + k];
+        mxk = wck;
+        mnk = wck;
+        for (int i = 0; i < C; ++i) {
+            const v2 M = sM[4 * lc + i];
+            const R cj = sWr[4 * sSt[sell(lc, i, maxS)] + k];
+            const R dphi = cj - wck;
+            gxk += M.x * dphi;
+            gyk += M.y * dphi;
+            mxk = fmax(cj, mxk);
+            mnk = fmin(cj, mnk);
+        }
+        v2* g = k == 0 ? p.gp : k == 1 ? p.gu : k == 2 ? p.gv : p.gT;
+        g[c] = v2{gxk, gyk};
+    };
     const int isNei = sf < 0;
     const int f = (sf < 0 ? -sf : sf) - 1;
     const bool live = k < C && f < nf;
@@ -399,36 +442,13 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         deltaR = (R)afar_hypot(e.x, e.y);
         fraction = sqrt(dx * dx + dy * dy) / deltaR;
     }
-
-    // least-squares gradient, from the staged weights (same order as the
-    // one-lane kernel); the full path forms it after the RBF sums, so that
-    // fewer values are live while the coefficients are in registers
-    R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
-    auto gradient = [&]() {
-        for (int i = 0; i < C; ++i) {
-            const v2 M = sM[4 * lc + i];
-            const v4 wj = sW[sSt[sell(lc, i, maxS)]];
-            R dphi = wj.x - wc.x;
-            gpx += M.x * dphi;
-            gpy += M.y * dphi;
-            dphi = wj.y - wc.y;
-            guxx += M.x * dphi;
-            guyx += M.y * dphi;
-            dphi = wj.z - wc.z;
-            guxy += M.x * dphi;
-            guyy += M.y * dphi;
-            dphi = wj.w - wc.w;
-            gTx += M.x * dphi;
-            gTy += M.y * dphi;
-        }
-        if (k == 0) p.gp[c] = v2{gpx, gpy};
-        if (k == 1) p.gu[c] = v2{guxx, guyx};
-        if (k == 2) p.gv[c] = v2{guxy, guyy};
-        if (k == 3) p.gT[c] = v2{gTx, gTy};
-    };
+    const v4 wc = sW[lc];
 
     if (flat) {
         gradient();
+        const R gpx = quad_bcast<0>(gxk), gpy = quad_bcast<0>(gyk), guxx = quad_bcast<1>(gxk),
+                guyx = quad_bcast<1>(gyk), guxy = quad_bcast<2>(gxk), guyy = quad_bcast<2>(gyk),
+                gTx = quad_bcast<3>(gxk), gTy = quad_bcast<3>(gyk);
         if (k == 0) {
             p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
             p.smooth[c] = 0;
@@ -465,16 +485,11 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         }
     }
     gradient();
-    R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z;
-    for (int i = 0; i < C; ++i) {
-        const v4 wj = sW[sSt[sell(lc, i, maxS)]];
-        pMax = fmax(wj.x, pMax);
-        pMin = fmin(wj.x, pMin);
-        uxMax = fmax(wj.y, uxMax);
-        uxMin = fmin(wj.y, uxMin);
-        uyMax = fmax(wj.z, uyMax);
-        uyMin = fmin(wj.z, uyMin);
-    }
+    const R gpx = quad_bcast<0>(gxk), gpy = quad_bcast<0>(gyk), guxx = quad_bcast<1>(gxk),
+            guyx = quad_bcast<1>(gyk), guxy = quad_bcast<2>(gxk), guyy = quad_bcast<2>(gyk),
+            gTx = quad_bcast<3>(gxk), gTy = quad_bcast<3>(gyk);
+    const R pMax = quad_bcast<0>(mxk), pMin = quad_bcast<0>(mnk), uxMax = quad_bcast<1>(mxk),
+            uxMin = quad_bcast<1>(mnk), uyMax = quad_bcast<2>(mxk), uyMin = quad_bcast<2>(mnk);
 
     R limP = 2, limUx = 2, limUy = 2, limT = 2;
     R smoothness = 0;
@@ -499,13 +514,16 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         p.fvT[2 * f + isNei] = o.w;
     }
     // group reductions (lanes 4j..4j+3 of the wavefront)
-    for (int off = 1; off < 4; off <<= 1) {
-        limP = fmin(limP, (R)__shfl_xor(limP, off));
-        limUx = fmin(limUx, (R)__shfl_xor(limUx, off));
-        limUy = fmin(limUy, (R)__shfl_xor(limUy, off));
-        limT = fmin(limT, (R)__shfl_xor(limT, off));
-        smoothness = fmax(smoothness, (R)__shfl_xor(smoothness, off));
-    }
+    limP = fmin(limP, quad_xor<1>(limP));
+    limUx = fmin(limUx, quad_xor<1>(limUx));
+    limUy = fmin(limUy, quad_xor<1>(limUy));
+    limT = fmin(limT, quad_xor<1>(limT));
+    smoothness = fmax(smoothness, quad_xor<1>(smoothness));
+    limP = fmin(limP, quad_xor<2>(limP));
+    limUx = fmin(limUx, quad_xor<2>(limUx));
+    limUy = fmin(limUy, quad_xor<2>(limUy));
+    limT = fmin(limT, quad_xor<2>(limT));
+    smoothness = fmax(smoothness, quad_xor<2>(smoothness));
     // the trouble test of the last internal face in slot order
     const unsigned long long lm = __ballot(live);
     const int base = (t & 63) & ~3;
