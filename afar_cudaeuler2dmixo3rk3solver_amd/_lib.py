@@ -33,6 +33,7 @@ class Schemes(C.Structure):
         ("tile", C.c_int32),
         ("boundaryFirstOrder", C.c_int32),
         ("reconLanes", C.c_int32),
+        ("cellOrder", C.c_int32),
     ]
 
 

@@ -110,6 +110,142 @@ RefArrays make_ref_arrays(const Mesh& m, const Precomputed& pc, const double* W0
     return ra;
 }
 
+namespace {
+
+// Index of grid point (x, y) along the Hilbert curve through an n x n grid
+// (n a power of two): the quadrant walk of Hilbert (1891), with the sub-square
+// rotated or reflected so that consecutive indices are neighbours.
+uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        const uint32_t rx = (x & s) ? 1u : 0u, ry = (y & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            const uint32_t t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+}  // namespace
+
+Relabel compact_tile_order(const RefArrays& ra, const BatchIndex& bi) {
+    const int nc = ra.nc, nf = ra.nf;
+    Relabel r;
+    r.cell_old.resize(nc);
+    r.cell_new.resize(nc);
+    std::vector<uint64_t> key(nc);
+    const uint32_t n = 1u << 16;
+    for (int k = 0; k < bi.ninst(); ++k) {
+        const int c0 = bi.cell_off[k], c1 = bi.cell_off[k + 1];
+        if (c1 <= c0) continue;
+        double x0 = ra.CELL[2 * c0], x1 = x0, y0 = ra.CELL[2 * c0 + 1], y1 = y0;
+        for (int c = c0; c < c1; ++c) {
+            x0 = std::min(x0, ra.CELL[2 * c]);
+            x1 = std::max(x1, ra.CELL[2 * c]);
+            y0 = std::min(y0, ra.CELL[2 * c + 1]);
+            y1 = std::max(y1, ra.CELL[2 * c + 1]);
+        }
+        // one scale for both axes keeps tiles square on elongated domains
+        const double ext = std::max(x1 - x0, y1 - y0);
+        const double sc = ext > 0.0 ? (double)(n - 1) / ext : 0.0;
+        for (int c = c0; c < c1; ++c) {
+            const uint32_t gx = (uint32_t)std::min((double)(n - 1), (ra.CELL[2 * c] - x0) * sc);
+            const uint32_t gy = (uint32_t)std::min((double)(n - 1), (ra.CELL[2 * c + 1] - y0) * sc);
+            key[c] = hilbert_index(n, gx, gy);
+            r.cell_old[c] = c;
+        }
+        std::sort(r.cell_old.begin() + c0, r.cell_old.begin() + c1,
+                  [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
+    }
+    for (int j = 0; j < nc; ++j) r.cell_new[r.cell_old[j]] = j;
+    r.face_old.resize(nf);
+    r.face_new.resize(nf);
+    for (int f = 0; f < nf; ++f) r.face_old[f] = f;
+    for (int k = 0; k < bi.ninst(); ++k) {
+        const int f0 = bi.face_off[k], f1 = bi.face_off[k + 1];
+        std::sort(r.face_old.begin() + f0, r.face_old.begin() + f1, [&](int a, int b) {
+            const int oa = r.cell_new[ra.neighbour[2 * a]], ob = r.cell_new[ra.neighbour[2 * b]];
+            return oa != ob ? oa < ob : a < b;
+        });
+    }
+    for (int j = 0; j < nf; ++j) r.face_new[r.face_old[j]] = j;
+    return r;
+}
+
+RefArrays relabel(const RefArrays& ra, const Relabel& r) {
+    const int nc = ra.nc, nf = ra.nf, nb = ra.nb, maxS = ra.maxS, maxC = ra.maxC;
+    RefArrays o;
+    o.nc = nc;
+    o.nf = nf;
+    o.nb = nb;
+    o.npatch = ra.npatch;
+    o.maxS = maxS;
+    o.maxC = maxC;
+    auto cell = [&](int32_t c) { return c < nc ? r.cell_new[c] : c; };  // pseudo-cells keep their ids
+    auto sface = [&](int32_t sf) {  // +-(f+1), boundary faces keep their ids
+        if (sf == 0) return sf;
+        const int f = (sf > 0 ? sf : -sf) - 1;
+        if (f >= nf) return sf;
+        const int32_t g = r.face_new[f] + 1;
+        return sf > 0 ? g : -g;
+    };
+    o.CELL = ra.CELL;
+    o.cellVolume.resize(nc);
+    o.FD = ra.FD;
+    o.cFD.resize(ra.cFD.size());
+    o.stencilSize.resize(nc);
+    o.compactStencilSize.resize(nc);
+    o.stencil.assign(ra.stencil.size(), 0);
+    o.cellFaces.assign(ra.cellFaces.size(), 0);
+    for (int j = 0; j < nc; ++j) {
+        const int c = r.cell_old[j];
+        o.CELL[2 * j] = ra.CELL[2 * c];
+        o.CELL[2 * j + 1] = ra.CELL[2 * c + 1];
+        o.cellVolume[j] = ra.cellVolume[c];
+        for (int k = 0; k < 4; ++k) o.FD[4 * (size_t)j + k] = ra.FD[4 * (size_t)c + k];
+        if (!ra.cFD.empty())
+            for (int k = 0; k < 4; ++k) o.cFD[4 * (size_t)j + k] = ra.cFD[4 * (size_t)c + k];
+        o.stencilSize[j] = ra.stencilSize[c];
+        o.compactStencilSize[j] = ra.compactStencilSize[c];
+        for (int i = 0; i < maxS; ++i)
+            o.stencil[(size_t)maxS * j + i] =
+                i < ra.stencilSize[c] ? cell(ra.stencil[(size_t)maxS * c + i]) : ra.stencil[(size_t)maxS * c + i];
+        for (int i = 0; i < maxC; ++i)
+            o.cellFaces[(size_t)maxC * j + i] =
+                i < ra.compactStencilSize[c] ? sface(ra.cellFaces[(size_t)maxC * c + i]) : 0;
+    }
+    o.neighbour.resize(2 * (size_t)nf);
+    o.FACE = ra.FACE;
+    o.FSHIFT.resize(ra.FSHIFT.size());
+    const size_t W = 2 * (size_t)(maxS + 1);
+    o.RBFbasis.resize(ra.RBFbasis.size());
+    for (int g = 0; g < nf; ++g) {
+        const int f = r.face_old[g];
+        o.neighbour[2 * g] = cell(ra.neighbour[2 * f]);
+        o.neighbour[2 * g + 1] = cell(ra.neighbour[2 * f + 1]);
+        for (int k = 0; k < 5; ++k) o.FACE[5 * (size_t)g + k] = ra.FACE[5 * (size_t)f + k];
+        if (!ra.FSHIFT.empty()) {
+            o.FSHIFT[2 * g] = ra.FSHIFT[2 * f];
+            o.FSHIFT[2 * g + 1] = ra.FSHIFT[2 * f + 1];
+        }
+        if (!ra.RBFbasis.empty())
+            std::copy(ra.RBFbasis.begin() + W * f, ra.RBFbasis.begin() + W * (f + 1), o.RBFbasis.begin() + W * g);
+    }
+    o.boundaryFacesNum = ra.boundaryFacesNum;
+    o.boundaryFaceNeiLabel.resize(nb);
+    for (int b = 0; b < nb; ++b) o.boundaryFaceNeiLabel[b] = cell(ra.boundaryFaceNeiLabel[b]);
+    o.boundaryFacesType = ra.boundaryFacesType;
+    o.BVALUE = ra.BVALUE;
+    return o;
+}
+
 RefArrays union_ref_arrays(const std::vector<const RefArrays*>& parts, BatchIndex& idx) {
     const int n = (int)parts.size();
     if (n == 0) throw Error(1, "empty batch");

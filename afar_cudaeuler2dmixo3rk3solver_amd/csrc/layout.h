@@ -86,6 +86,22 @@ struct BatchIndex {
 };
 RefArrays union_ref_arrays(const std::vector<const RefArrays*>& parts, BatchIndex& idx);
 
+// Device order of cells and internal faces.  The mesh's order (reverse
+// Cuthill-McKee) makes consecutive cells a thin front, so a 64-cell tile has
+// about four halo cells per cell for an extended stencil.  Cells ordered along
+// a Hilbert curve through their centres form compact tiles instead; internal
+// faces then follow their owner's new index.  Every instance of a batch keeps
+// its cell and face ranges.  Relabelling changes no arithmetic: each cell
+// keeps its stencil and face lists in their order.
+struct Relabel {
+    std::vector<int32_t> cell_new, cell_old;  // nc: new index of mesh cell i; mesh cell at new index j
+    std::vector<int32_t> face_new, face_old;  // nf internal faces, likewise
+};
+Relabel compact_tile_order(const RefArrays& ra, const BatchIndex& bi);
+// The same problem with cells and internal faces renamed by r (pseudo-cells
+// and boundary faces keep their indices).
+RefArrays relabel(const RefArrays& ra, const Relabel& r);
+
 // The reference's 128-cell block tables (basicCUDAsolver.cpp:697-850): per
 // block the halo cells (compact-stencil halo first, then the rest of the
 // extended stencil) and block-local stencil ids.

@@ -5,7 +5,9 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cstdio>
+#include <string>
 #include <mutex>
 
 #include "kernels.h"
@@ -542,13 +544,156 @@ class SolverImpl final : public DeviceSolver {
         m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res;
 };
 
+// A solver on relabelled cells and faces (compact_tile_order) that reads and
+// writes every cell- and face-indexed array in the caller's order.
+class RelabeledSolver final : public DeviceSolver {
+   public:
+    RelabeledSolver(std::unique_ptr<DeviceSolver> in, Relabel r, int nc, int nf, int nb)
+        : in_(std::move(in)), r_(std::move(r)), nc_(nc), nf_(nf), nb_(nb) {
+        cfg_ = in_->config();
+    }
+    double compute_dt() override { return in_->compute_dt(); }
+    void evolve(double dt) override { in_->evolve(dt); }
+    void evolve_auto(int nsteps) override { in_->evolve_auto(nsteps); }
+    void synchronize() override { in_->synchronize(); }
+    bool idle() override { return in_->idle(); }
+    void copy_out(double* FD, int8_t* shock) override {
+        std::vector<double> f(FD ? 4 * (size_t)(nc_ + nb_) : 0);
+        std::vector<int8_t> s(shock ? 2 * (size_t)(nc_ + nb_) : 0);
+        in_->copy_out(FD ? f.data() : nullptr, shock ? s.data() : nullptr);
+        for (int i = 0; i < nc_ + nb_; ++i) {
+            const size_t j = i < nc_ ? (size_t)r_.cell_new[i] : (size_t)i;
+            if (FD)
+                for (int k = 0; k < 4; ++k) FD[4 * (size_t)i + k] = f[4 * j + k];
+            if (shock) {
+                shock[2 * (size_t)i] = s[2 * j];
+                shock[2 * (size_t)i + 1] = s[2 * j + 1];
+            }
+        }
+    }
+    void get_conserved(double* U) override {
+        std::vector<double> u(4 * (size_t)nc_);
+        in_->get_conserved(u.data());
+        for (int i = 0; i < nc_; ++i)
+            for (int k = 0; k < 4; ++k) U[4 * (size_t)i + k] = u[4 * (size_t)r_.cell_new[i] + k];
+    }
+    void set_conserved(const double* U) override {
+        std::vector<double> u(4 * (size_t)nc_);
+        for (int i = 0; i < nc_; ++i)
+            for (int k = 0; k < 4; ++k) u[4 * (size_t)r_.cell_new[i] + k] = U[4 * (size_t)i + k];
+        in_->set_conserved(u.data());
+    }
+    double last_dt() override { return in_->last_dt(); }
+    int ninst() const override { return in_->ninst(); }
+    void last_dts(double* out) override { in_->last_dts(out); }
+    void get_conserved_range(int c0, int n, double* U) override {
+        int lo, hi;
+        span(c0, n, lo, hi);
+        std::vector<double> u(4 * (size_t)(hi - lo));
+        if (hi > lo) in_->get_conserved_range(lo, hi - lo, u.data());
+        for (int i = 0; i < n; ++i)
+            for (int k = 0; k < 4; ++k) U[4 * (size_t)i + k] = u[4 * (size_t)(r_.cell_new[c0 + i] - lo) + k];
+    }
+    void get_primitive_range(int c0, int n, double* W, int8_t* type) override {
+        int lo, hi;
+        span(c0, n, lo, hi);
+        std::vector<double> w(W ? 4 * (size_t)(hi - lo) : 0);
+        std::vector<int8_t> t(type ? (size_t)(hi - lo) : 0);
+        if (hi > lo) in_->get_primitive_range(lo, hi - lo, W ? w.data() : nullptr, type ? t.data() : nullptr);
+        for (int i = 0; i < n; ++i) {
+            const size_t j = (size_t)(r_.cell_new[c0 + i] - lo);
+            if (W)
+                for (int k = 0; k < 4; ++k) W[4 * (size_t)i + k] = w[4 * j + k];
+            if (type) type[i] = t[j];
+        }
+    }
+    void reset_state() override { in_->reset_state(); }
+    void set_time(int inst, double t, double t_end) override { in_->set_time(inst, t, t_end); }
+    void get_time(double* t, double* steps) override { in_->get_time(t, steps); }
+    double residual_norm() override { return in_->residual_norm(); }
+    double time_steps(int nsteps, bool graph) override { return in_->time_steps(nsteps, graph); }
+    void profile(double* ms, int* count, int reps) override { in_->profile(ms, count, reps); }
+    const DevLayout& layout() const override { return in_->layout(); }
+    // Cell arrays (W, U, gradients, lim, smooth, ftype) and face arrays (fv,
+    // flux) come back in the caller's order; the slot arrays (lsq) stay in
+    // the device's.
+    void debug_fetch(const char* what, void* out, size_t bytes) override {
+        std::vector<unsigned char> b(bytes);
+        in_->debug_fetch(what, b.data(), bytes);
+        unsigned char* o = static_cast<unsigned char*>(out);
+        const std::string w(what);
+        auto cells = [&](int n) {  // n entries: nc cells (then pseudo-cells)
+            const size_t e = bytes / n;
+            for (int i = 0; i < n; ++i) {
+                const size_t j = i < nc_ ? (size_t)r_.cell_new[i] : (size_t)i;
+                memcpy(o + e * i, b.data() + e * j, e);
+            }
+        };
+        auto faces = [&](int n, int per) {  // n entries of `per` records: nf faces (then boundary faces)
+            const size_t e = bytes / ((size_t)n * per) * per;
+            for (int i = 0; i < n; ++i) {
+                const size_t j = i < nf_ ? (size_t)r_.face_new[i] : (size_t)i;
+                memcpy(o + e * i, b.data() + e * j, e);
+            }
+        };
+        if (w == "W" || w == "smooth")
+            cells(nc_ + nb_);
+        else if (w == "U" || w == "gp" || w == "gu" || w == "gv" || w == "gT" || w == "lim" || w == "ftype")
+            cells(nc_);
+        else if (w == "fv")
+            faces(nf_, 2);
+        else if (w == "flux")
+            faces(nf_ + nb_, 1);
+        else
+            memcpy(out, b.data(), bytes);
+    }
+    void run_stage_kernels(int upto, double dt, int stage) override { in_->run_stage_kernels(upto, dt, stage); }
+    void set_constants(double R, double Cv, double CFL) override {
+        in_->set_constants(R, Cv, CFL);
+        cfg_ = in_->config();
+    }
+
+   private:
+    // the device range holding mesh cells [c0, c0+n)
+    void span(int c0, int n, int& lo, int& hi) const {
+        if (c0 < 0 || n < 0 || c0 + n > nc_) throw Error(1, "cell range out of bounds");
+        lo = nc_;
+        hi = 0;
+        for (int i = c0; i < c0 + n; ++i) {
+            lo = std::min(lo, r_.cell_new[i]);
+            hi = std::max(hi, r_.cell_new[i] + 1);
+        }
+        if (hi < lo) lo = hi = 0;
+    }
+    std::unique_ptr<DeviceSolver> in_;
+    Relabel r_;
+    int nc_, nf_, nb_;
+};
+
+std::unique_ptr<DeviceSolver> make_impl(const RefArrays& ra, const SolverConfig& cfg, const BatchIndex* bi) {
+    if (cfg.precision == 1) return std::unique_ptr<DeviceSolver>(new SolverImpl<float>(ra, cfg, bi));
+    return std::unique_ptr<DeviceSolver>(new SolverImpl<double>(ra, cfg, bi));
+}
+
 }  // namespace
 
 std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg,
                                                  const BatchIndex* bi) {
     if (ra.nc <= 0) throw Error(1, "empty mesh");
-    if (cfg.precision == 1) return std::unique_ptr<DeviceSolver>(new SolverImpl<float>(ra, cfg, bi));
-    return std::unique_ptr<DeviceSolver>(new SolverImpl<double>(ra, cfg, bi));
+    if (cfg.cellOrder == 1) return make_impl(ra, cfg, bi);
+    BatchIndex one;
+    if (!bi) {
+        one.cell_off = {0, ra.nc};
+        one.face_off = {0, ra.nf};
+        one.bface_off = {0, ra.nb};
+    }
+    Relabel r = compact_tile_order(ra, bi ? *bi : one);
+    std::unique_ptr<DeviceSolver> in;
+    {
+        const RefArrays rl = relabel(ra, r);
+        in = make_impl(rl, cfg, bi);
+    }
+    return std::unique_ptr<DeviceSolver>(new RelabeledSolver(std::move(in), std::move(r), ra.nc, ra.nf, ra.nb));
 }
 
 }  // namespace afar

@@ -18,6 +18,7 @@ struct SolverConfig {
     int tile = 64;
     int boundaryFirstOrder = 0;
     int reconLanes = 0;  // reconstruction kernel: 0 automatic, 1 one lane per cell, 2 compacted, 4 four lanes
+    int cellOrder = 0;   // device cell order: 0 compact tiles (Hilbert curve), 1 the mesh's order
 };
 
 // kernel kinds reported by profile()

@@ -19,9 +19,9 @@ STENCIL_TYPES = {"faceNeighbour": 0, "edgeNeighbour": 1}
 
 def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilType="edgeNeighbour",
             recalcBasis=True, cOrder=5, fOrder=4, precision="fp64", exactRK=False, tile=64,
-            boundaryFirstOrder=False, reconLanes=0):
+            boundaryFirstOrder=False, reconLanes=0, cellOrder=0):
     """afar_schemes with the reference's key names and defaults (the last
-    five are this library's extensions)."""
+    six are this library's extensions)."""
     s = Schemes()
     lib().afar_schemes_default(C.byref(s))
     s.R, s.Cv, s.maxCo, s.shapeFactor = R, Cv, maxCo, shapeFactor
@@ -33,6 +33,7 @@ def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilTy
     s.tile = tile
     s.boundaryFirstOrder = int(bool(boundaryFirstOrder))
     s.reconLanes = int(reconLanes)
+    s.cellOrder = int(cellOrder)
     return s
 
 

@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--recon-lanes", type=int, default=0, choices=[0, 1, 2, 4],
                     help="reconstruction kernel: 1 one thread per cell, 4 four per cell, 2 four per cell with "
                          "the face work compacted; 0 automatic")
+    ap.add_argument("--cell-order", type=int, default=0, choices=[0, 1],
+                    help="device cell order: 0 compact tiles along a Hilbert curve, 1 the mesh's (RCM) order")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--faithful-boundary", action="store_true",
                     help="riemann case without the boundaryFirstOrder extension (loses positivity)")
@@ -251,7 +253,7 @@ def main():
     # the boundaryFirstOrder extension so the timed steps act on a physical state.
     bfo = args.case == "riemann" and not args.faithful_boundary
     sc = A.schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=args.precision, tile=args.tile, boundaryFirstOrder=bfo,
-                   reconLanes=args.recon_lanes)
+                   reconLanes=args.recon_lanes, cellOrder=args.cell_order)
     mesh.precompute(sc)
     t_pre = time.perf_counter() - t0
     print("bench: %s, nc=%d, precompute %.1f s" % (args.case, mesh.nc, t_pre), file=sys.stderr, flush=True)
@@ -322,7 +324,8 @@ def main():
         "config": {"workload": workload, "case": args.case, "cells": mesh.nc, "internal_faces": st["nf"],
                    "boundary_faces": st["nb"], "mean_stencil": round(st["meanS"], 3),
                    "mean_compact": round(st["meanC"], 3), "precision": args.precision, "cfl": 0.5,
-                   "tile": args.tile, "recon_lanes": args.recon_lanes or "auto", "device": dev_name.strip(),
+                   "tile": args.tile, "recon_lanes": args.recon_lanes or "auto",
+                   "cell_order": ["compact tiles (Hilbert)", "mesh (RCM)"][args.cell_order], "device": dev_name.strip(),
                    "precompute_s": round(t_pre, 1),
                    "boundary_first_order": bfo, "state_positive_after_timed_steps": finite,
                    "t_start": t_start, "untimed_steps_to_t_start": pre_steps, "t_after_timed_steps": round(t_end, 5),

@@ -65,6 +65,9 @@ typedef struct afar_schemes {
                             cell, 4 four threads per cell, 2 four threads per cell with the
                             tile's face work compacted (2 and 4: cells of at most four faces;
                             2: tiles of at most 128 cells; automatic picks 4, else 1) */
+    int32_t cellOrder;   /* device order of cells, same results: 0 compact tiles (cells along a
+                            Hilbert curve through their centres, faces by owner), 1 the mesh's
+                            order; every array the library returns is in the mesh's order */
 } afar_schemes;
 
 const char* afar_last_error(void);
