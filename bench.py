@@ -367,7 +367,10 @@ def run_batch(args):
     from afar_cudaeuler2dmixo3rk3solver_amd import batch as Bt
     log = lambda m: print("bench: " + m, file=sys.stderr, flush=True)
     precision = "mixed" if args.precision == "fp64" and not args.batch_fp64 else args.precision
-    sc_kwargs = dict(cOrder=7, fOrder=2, maxCo=0.5, precision=precision)
+    # With the reference's boundary treatment 18 of the 256 flows (inflow Mach
+    # above about 3 with the steeper corners) lose positivity within 100
+    # steps; the boundaryFirstOrder extension keeps every one physical.
+    sc_kwargs = dict(cOrder=7, fOrder=2, maxCo=0.5, precision=precision, boundaryFirstOrder=True)
     t0 = time.perf_counter()
     params = Bt.corner_instances(n=args.batch_size, seed=2)
     meshes, W0s = Bt.build_corner_instances(params, A.schemes(**sc_kwargs), log=log)
@@ -387,7 +390,9 @@ def run_batch(args):
     wall = time.perf_counter() - w0
     cells = bb.cells
     value = cells * args.steps / wall / 1e6
-    finite = all(np.isfinite(b.conserved(k)).all() for _, b in bb.batches for k in (0, len(b.meshes) - 1))
+    def physical(U):
+        return bool(np.isfinite(U).all() and (U[:, 0] > 0).all())
+    n_physical = sum(physical(b.conserved(k)) for _, b in bb.batches for k in range(len(b.meshes)))
     out = {
         "metric": "Mcell-RK3-steps/s", "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
@@ -397,7 +402,8 @@ def run_batch(args):
                                % len(meshes), "case": "batch", "meshes": len(meshes), "cells": cells,
                    "precision": precision, "cfl": 0.5, "device": A.device_info()[0].strip(),
                    "precompute_s": round(t_pre, 1), "autotune_s": round(t_tune, 1),
-                   "buckets": {str(k): v for k, v in bb.tuning.items()}, "state_finite": bool(finite),
+                   "buckets": {str(k): v for k, v in bb.tuning.items()}, "boundary_first_order": True,
+                   "meshes_finite_positive_after_timed_steps": n_physical,
                    "timing": "host wall clock around the enqueue of all buckets' steps and their completion"},
         "roofline": None, "cpu_baseline": None,
     }
