@@ -34,6 +34,7 @@ class Schemes(C.Structure):
         ("boundaryFirstOrder", C.c_int32),
         ("reconLanes", C.c_int32),
         ("cellOrder", C.c_int32),
+        ("faceQuadrature", C.c_int32),
     ]
 
 
@@ -102,6 +103,7 @@ SIGNATURES = {
     "afar_mesh_precompute_gpu": ([_VP, P(Schemes)], C.c_int),
     "afar_mesh_stencils": ([_VP, _U8, _U8, _I32, _I32], C.c_int),
     "afar_mesh_rbf_basis": ([_VP, _D, _U8, _D], C.c_int),
+    "afar_mesh_face_quadrature": ([_VP, C.POINTER(C.c_int32), _D, _D, _D, _D], C.c_int),
     "afar_mesh_gauss_points": ([_VP, C.c_int32, C.c_int32, _D, _D, C.c_int32, _I32], C.c_int),
     "afar_mesh_write_basis": ([_VP, C.c_char_p], C.c_int),
     "afar_pinv": ([C.c_int32, _D, _D], C.c_int),

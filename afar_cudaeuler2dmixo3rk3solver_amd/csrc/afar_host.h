@@ -95,6 +95,7 @@ struct Schemes {
     int cOrder = 5, fOrder = 4;  // reference defaults (d2GaussQuadrature.C:12-13)
     int precision = 0;           // 0 FP64, 1 FP32-mixed
     int exactRK = 0;             // 0 literal reference coefficients, 1 exact 1/3, 2/3
+    int faceQuadrature = 0;      // extension: 0 flux at the face centre (reference), 1 at the fOrder points
 };
 
 // Reconstruction stencils (src/CUDA/cudaEuler2DmixO3RK3/basicCUDAsolver.cpp:269-417).
@@ -112,6 +113,12 @@ struct Precomputed {
     std::vector<uint8_t> trouble;  // per cell: singular moment matrix
     std::vector<double> refDistance;
     int troubleCount = 0;
+    // extension (faceQuadrature 1): flux quadrature points of every internal
+    // face, xi on [-1, 1] with weights summing to one, and each face side's
+    // RBF coefficients at each point: rbf_gp[(g*nf + f)*2*(maxS+1) + side*(maxS+1) + i]
+    int ngp = 0;
+    std::vector<double> gp_xi, gp_w;
+    std::vector<double> rbf_gp;
 };
 
 // ---- mesh.cpp
@@ -126,6 +133,10 @@ std::vector<int32_t> rcm_order(int n, const std::vector<int32_t>& adj_ptr, const
 Stencils build_stencils(const Mesh& m, int stencilType);
 void gauss_cell(const Mesh& m, int cell, int cOrder, std::vector<Vec2>& pts, std::vector<double>& w);
 void gauss_boundary_face(const Mesh& m, int b, int fOrder, std::vector<Vec2>& pts, std::vector<double>& w);
+// The fOrder bar rule on [-1, 1] (d2GaussQuadrature.C:489-547) and a point of
+// face f at xi, in the owner's frame (d2GaussQuadrature.C:199-206).
+void face_gauss_rule(int fOrder, std::vector<double>& xi, std::vector<double>& w);
+Vec2 face_gauss_point(const Mesh& m, int f, double xi);
 void pinv_jacobi(int n, const double* A, double* G);  // row-major n x n, n <= 21
 void gauss_tables(const Mesh& m, const Schemes& sc, std::vector<double>& pts, std::vector<double>& w,
                   std::vector<int64_t>& ptr);

@@ -54,7 +54,9 @@ Schemes to_schemes(const afar_schemes* s) {
     sc.fOrder = s->fOrder;
     sc.precision = s->precision;
     sc.exactRK = s->exactRK;
+    sc.faceQuadrature = s->faceQuadrature;
     need(sc.R > 0 && sc.Cv > 0, "R and Cv must be positive");
+    need(sc.faceQuadrature == 0 || sc.faceQuadrature == 1, "faceQuadrature must be 0 or 1");
     need(sc.precision == 0 || sc.precision == 1, "precision must be 0 (FP64) or 1 (FP32-mixed)");
     return sc;
 }
@@ -359,6 +361,8 @@ int afar_mesh_precompute(afar_mesh* h, const afar_schemes* s, const char* basis_
     return guard([&] {
         need(h && s, "NULL argument");
         Schemes sc = to_schemes(s);
+        need(!(sc.faceQuadrature && !sc.recalcBasis),
+             "faceQuadrature needs recalcBasis true (the basis cache holds face-centre coefficients only)");
         h->pc = precompute(h->m, sc);
         if (!sc.recalcBasis) {
             if (!basis_dir) throw Error(AFAR_ERR_IO, "recalcBasis false needs the basis directory");
@@ -425,6 +429,34 @@ int afar_mesh_rbf_basis(const afar_mesh* h, double* rbf, uint8_t* trouble, doubl
     });
 }
 
+int afar_mesh_face_quadrature(const afar_mesh* h, int32_t* ngp, double* xi, double* weights, double* offsets,
+                              double* rbf) {
+    return guard([&] {
+        need(h && h->precomputed && ngp, "mesh not precomputed or ngp NULL");
+        const Precomputed& pc = h->pc;
+        *ngp = pc.ngp;
+        if (!pc.ngp) return;
+        if (xi) memcpy(xi, pc.gp_xi.data(), sizeof(double) * pc.ngp);
+        if (weights) memcpy(weights, pc.gp_w.data(), sizeof(double) * pc.ngp);
+        if (rbf) memcpy(rbf, pc.rbf_gp.data(), sizeof(double) * pc.rbf_gp.size());
+        if (offsets) {
+            // the offsets the solver uses (make_ref_arrays), without a state
+            const Mesh& m = h->m;
+            for (int g = 0; g < pc.ngp; ++g)
+                for (int f = 0; f < m.nf; ++f) {
+                    const Vec2 x = face_gauss_point(m, f, pc.gp_xi[g]);
+                    const int own = m.own[f], nei = m.nei[f];
+                    const double xn = m.C[nei].x + m.fshift[f].x, yn = m.C[nei].y + m.fshift[f].y;
+                    double* d = &offsets[4 * ((size_t)g * m.nf + f)];
+                    d[0] = x.x - m.C[own].x;
+                    d[1] = x.y - m.C[own].y;
+                    d[2] = x.x - xn;
+                    d[3] = x.y - yn;
+                }
+        }
+    });
+}
+
 int afar_mesh_gauss_points(const afar_mesh* h, int32_t member, int32_t order, double* xy, double* w, int32_t cap,
                            int32_t* count) {
     return guard([&] {
@@ -472,6 +504,8 @@ static RefArrays ref_arrays_of(const afar_mesh* h, const afar_schemes* s, const 
     need(h && h->precomputed, "mesh not precomputed");
     need(W0 != nullptr, "W0 is NULL");
     Schemes sc = to_schemes(s);
+    need((sc.faceQuadrature != 0) == (h->pc.ngp > 0),
+         "faceQuadrature of the solver differs from the mesh precompute's; precompute with the same schemes");
     const Mesh& m = h->m;
     std::vector<double> w(4 * (size_t)m.nc);
     for (int i = 0; i < m.nc; ++i)

@@ -51,6 +51,12 @@ struct DevPtrs {
     const double4* fgeo;       // 2*nf: (Sfx, Sfy, magSf, deltaR), (dxo, dyo, dxn, dyn)
     typename VT<R>::v4* fv;    // 2*nf reconstructed face values, [2f + side]
     R* fvT;                    // 2*nf reconstructed face temperatures (the part the BVD test reads)
+    // extension (faceQuadrature): flux at ngp points per internal face
+    int ngp;                   // 0: face centre only (reference)
+    double gpw[4];             // point weights, sum 1
+    const R* rbf_gp;           // sliced ELL, RBF coefficients (point, face slot, stencil member)
+    const double4* fgd;        // ngp*nf: point - owner centre (x, y), point - neighbour centre (z, w)
+    typename VT<R>::v4* fvg;   // ngp*2*nf reconstructed states at the points, [(g*nf + f)*2 + side]
     double4* flux;             // nf + nb
     const int32_t* bcell;      // nb
     const uint8_t* blabel;     // nb

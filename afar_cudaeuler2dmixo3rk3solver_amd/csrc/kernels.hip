@@ -236,6 +236,16 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
             o.w = wc.w + gTx * dx + gTy * dy;
             p.fv[2 * f + isNei] = o;
             p.fvT[2 * f + isNei] = o.w;
+            for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
+                const double4 dg = p.fgd[(size_t)g * nf + f];
+                const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
+                v4 og;
+                og.x = wc.x + gpx * gx + gpy * gy;
+                og.y = wc.y + guxx * gx + guyx * gy;
+                og.z = wc.z + guxy * gx + guyy * gy;
+                og.w = wc.w + gTx * gx + gTy * gy;
+                p.fvg[((size_t)g * nf + f) * 2 + isNei] = og;
+            }
         }
         p.smooth[c] = 0;
         return;
@@ -283,7 +293,28 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
         o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
         isNotTrouble = phif > (R)0;
         p.fv[2 * f + isNei] = o;
-            p.fvT[2 * f + isNei] = o.w;
+        p.fvT[2 * f + isNei] = o.w;
+        for (int g = 0; g < p.ngp; ++g) {  // flux points (extension), as in k_reconstruct_q
+            const R* ag = p.rbf_gp + sell(c, (g * maxC + fi) * (maxS + 1), p.ngp * KR);
+            R cg = ag[0];
+            R pg = cg * wc.x, uxg = cg * wc.y, uyg = cg * wc.z, Tg = cg * wc.w;
+            for (int ci = 0; ci < S; ++ci) {
+                const v4 wj = sW[sSt[sell(tid, ci, maxS)]];
+                cg = ag[(size_t)(ci + 1) * 64];
+                pg += cg * wj.x;
+                uxg += cg * wj.y;
+                uyg += cg * wj.z;
+                Tg += cg * wj.w;
+            }
+            const double4 dg = p.fgd[(size_t)g * nf + f];
+            const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
+            v4 og;
+            og.x = (R)(rP >= K<R>::SMALL) * pg + (R)(rP < K<R>::SMALL) * (wc.x + (gpx * gx + gpy * gy));
+            og.y = (R)(rUx >= K<R>::SMALL) * uxg + (R)(rUx < K<R>::SMALL) * (wc.y + (guxx * gx + guyx * gy));
+            og.z = (R)(rUy >= K<R>::SMALL) * uyg + (R)(rUy < K<R>::SMALL) * (wc.z + (guxy * gx + guyy * gy));
+            og.w = (R)(rT >= K<R>::SMALL) * Tg + (R)(rT < K<R>::SMALL) * (wc.w + (gTx * gx + gTy * gy));
+            p.fvg[((size_t)g * nf + f) * 2 + isNei] = og;
+        }
     }
     const R nt = (R)isNotTrouble;
     p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
@@ -461,6 +492,16 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
             o.w = wc.w + gTx * dx + gTy * dy;
             p.fv[2 * f + isNei] = o;
             p.fvT[2 * f + isNei] = o.w;
+            for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
+                const double4 dg = p.fgd[(size_t)g * nf + f];
+                const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
+                v4 og;
+                og.x = wc.x + gpx * gx + gpy * gy;
+                og.y = wc.y + guxx * gx + guyx * gy;
+                og.z = wc.z + guxy * gx + guyy * gy;
+                og.w = wc.w + gTx * gx + gTy * gy;
+                p.fvg[((size_t)g * nf + f) * 2 + isNei] = og;
+            }
         }
         return;
     }
@@ -512,6 +553,29 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         faceNotTrouble = phif > (R)0;
         p.fv[2 * f + isNei] = o;
         p.fvT[2 * f + isNei] = o.w;
+        // flux points (extension): RBF sums with the point's coefficients,
+        // or the unlimited linear state where the compact range is flat
+        for (int g = 0; g < p.ngp; ++g) {
+            const R* ag = p.rbf_gp + sell(c, (g * maxC + k) * (maxS + 1), p.ngp * KR);
+            R cg = ag[0];
+            R pg = cg * wc.x, uxg = cg * wc.y, uyg = cg * wc.z, Tg = cg * wc.w;
+            for (int ci = 0; ci < S; ++ci) {
+                const v4 wj = sW[sSt[sell(lc, ci, maxS)]];
+                cg = ag[(size_t)(ci + 1) * 64];
+                pg += cg * wj.x;
+                uxg += cg * wj.y;
+                uyg += cg * wj.z;
+                Tg += cg * wj.w;
+            }
+            const double4 dg = p.fgd[(size_t)g * nf + f];
+            const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
+            v4 og;
+            og.x = (R)(rP >= K<R>::SMALL) * pg + (R)(rP < K<R>::SMALL) * (wc.x + (gpx * gx + gpy * gy));
+            og.y = (R)(rUx >= K<R>::SMALL) * uxg + (R)(rUx < K<R>::SMALL) * (wc.y + (guxx * gx + guyx * gy));
+            og.z = (R)(rUy >= K<R>::SMALL) * uyg + (R)(rUy < K<R>::SMALL) * (wc.z + (guxy * gx + guyy * gy));
+            og.w = (R)(rT >= K<R>::SMALL) * Tg + (R)(rT < K<R>::SMALL) * (wc.w + (gTx * gx + gTy * gy));
+            p.fvg[((size_t)g * nf + f) * 2 + isNei] = og;
+        }
     }
     // group reductions (lanes 4j..4j+3 of the wavefront)
     limP = fmin(limP, quad_xor<1>(limP));
@@ -987,50 +1051,83 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
         const v4 lL = p.lim[own], lR = p.lim[nei];
         const v2 gpL = p.gp[own], guL = p.gu[own], gvL = p.gv[own], gTL = p.gT[own];
         const v2 gpR = p.gp[nei], guR = p.gu[nei], gvR = p.gv[nei], gTR = p.gT[nei];
-        const v4 fL = p.fv[2 * f], fR = p.fv[2 * f + 1];
         if (DT) {
             const double d = cfl_face_dt(g0, g1, wL, wR, Rgas, Cv, CFL);
             dtf = d < dtf ? d : dtf;
         }
 
-        const R pLT = wL.x + lL.x * (gpL.x * dxL + gpL.y * dyL);
-        const R uxLT = wL.y + lL.y * (guL.x * dxL + guL.y * dyL);
-        const R uyLT = wL.z + lL.z * (gvL.x * dxL + gvL.y * dyL);
-        const R TLT = wL.w + lL.w * (gTL.x * dxL + gTL.y * dyL);
-        const R rhoLT = pLT / (Rg * TLT);
-        const R pRT = wR.x + lR.x * (gpR.x * dxR + gpR.y * dyR);
-        const R uxRT = wR.y + lR.y * (guR.x * dxR + guR.y * dyR);
-        const R uyRT = wR.z + lR.z * (gvR.x * dxR + gvR.y * dyR);
-        const R TRT = wR.w + lR.w * (gTR.x * dxR + gTR.y * dyR);
-        const R rhoRT = pRT / (Rg * TRT);
+        // the flux of the two states reconstructed at a point with offsets
+        // (dxL, dyL) from the owner and (dxR, dyR) from the neighbour; fL, fR
+        // are the high-order (O3) states there
+        auto point_flux = [&](R dxL, R dyL, R dxR, R dyR, const v4& fL, const v4& fR, R& out0, R& out1, R& out2,
+                              R& out3) {
+            const R pLT = wL.x + lL.x * (gpL.x * dxL + gpL.y * dyL);
+            const R uxLT = wL.y + lL.y * (guL.x * dxL + guL.y * dyL);
+            const R uyLT = wL.z + lL.z * (gvL.x * dxL + gvL.y * dyL);
+            const R TLT = wL.w + lL.w * (gTL.x * dxL + gTL.y * dyL);
+            const R rhoLT = pLT / (Rg * TLT);
+            const R pRT = wR.x + lR.x * (gpR.x * dxR + gpR.y * dyR);
+            const R uxRT = wR.y + lR.y * (guR.x * dxR + guR.y * dyR);
+            const R uyRT = wR.z + lR.z * (gvR.x * dxR + gvR.y * dyR);
+            const R TRT = wR.w + lR.w * (gTR.x * dxR + gTR.y * dyR);
+            const R rhoRT = pRT / (Rg * TRT);
 
-        const R pL = (R)(1 - isTVD_L) * fL.x + (R)isTVD_L * pLT;
-        const R uxL = (R)(1 - isTVD_L) * fL.y + (R)isTVD_L * uxLT;
-        const R uyL = (R)(1 - isTVD_L) * fL.z + (R)isTVD_L * uyLT;
-        const R TL = (R)(1 - isTVD_L) * fL.w + (R)isTVD_L * TLT;
-        const R rhoL = pL / (Rg * TL);
-        const R pR = (R)(1 - isTVD_R) * fR.x + (R)isTVD_R * pRT;
-        const R uxR = (R)(1 - isTVD_R) * fR.y + (R)isTVD_R * uxRT;
-        const R uyR = (R)(1 - isTVD_R) * fR.z + (R)isTVD_R * uyRT;
-        const R TR = (R)(1 - isTVD_R) * fR.w + (R)isTVD_R * TRT;
-        const R rhoR = pR / (Rg * TR);
+            const R pL = (R)(1 - isTVD_L) * fL.x + (R)isTVD_L * pLT;
+            const R uxL = (R)(1 - isTVD_L) * fL.y + (R)isTVD_L * uxLT;
+            const R uyL = (R)(1 - isTVD_L) * fL.z + (R)isTVD_L * uyLT;
+            const R TL = (R)(1 - isTVD_L) * fL.w + (R)isTVD_L * TLT;
+            const R rhoL = pL / (Rg * TL);
+            const R pR = (R)(1 - isTVD_R) * fR.x + (R)isTVD_R * pRT;
+            const R uxR = (R)(1 - isTVD_R) * fR.y + (R)isTVD_R * uxRT;
+            const R uyR = (R)(1 - isTVD_R) * fR.z + (R)isTVD_R * uyRT;
+            const R TR = (R)(1 - isTVD_R) * fR.w + (R)isTVD_R * TRT;
+            const R rhoR = pR / (Rg * TR);
 
-        const R dP = minimize_delta(pL, pR, pLT, pRT);
-        const R dRho = minimize_delta(rhoL, rhoR, rhoLT, rhoRT);
+            const R dP = minimize_delta(pL, pR, pLT, pRT);
+            const R dRho = minimize_delta(rhoL, rhoR, rhoLT, rhoRT);
 
-        const R CvR = (R)Cv;
-        R rhoFlux = (R)0.5 * rhoL * (uxL * Sfx + uyL * Sfy) + (R)0.5 * rhoR * (uxR * Sfx + uyR * Sfy);
-        R rhoUxFlux = (R)0.5 * (rhoFlux * uxL + pL * Sfx) + (R)0.5 * (rhoFlux * uxR + pR * Sfx);
-        R rhoUyFlux = (R)0.5 * (rhoFlux * uyL + pL * Sfy) + (R)0.5 * (rhoFlux * uyR + pR * Sfy);
-        R rhoEFlux = (R)0.5 * rhoFlux * (CvR * TL + (R)0.5 * (uxL * uxL + uyL * uyL) + pL / rhoL) +
-                     (R)0.5 * rhoFlux * (CvR * TR + (R)0.5 * (uxR * uxR + uyR * uyR) + pR / rhoR);
-        if (isShock)
-            kfvs_flux<R>(rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Rg,
-                         CvR, magSf, Sfx, Sfy);
-        else
-            roe_flux<R>(rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, dP,
-                        dRho, (R)((Cv + Rgas) / Cv), CvR, magSf, Sfx, Sfy);
-        p.flux[f] = double4{(double)rhoFlux, (double)rhoUxFlux, (double)rhoUyFlux, (double)rhoEFlux};
+            const R CvR = (R)Cv;
+            R rhoFlux = (R)0.5 * rhoL * (uxL * Sfx + uyL * Sfy) + (R)0.5 * rhoR * (uxR * Sfx + uyR * Sfy);
+            R rhoUxFlux = (R)0.5 * (rhoFlux * uxL + pL * Sfx) + (R)0.5 * (rhoFlux * uxR + pR * Sfx);
+            R rhoUyFlux = (R)0.5 * (rhoFlux * uyL + pL * Sfy) + (R)0.5 * (rhoFlux * uyR + pR * Sfy);
+            R rhoEFlux = (R)0.5 * rhoFlux * (CvR * TL + (R)0.5 * (uxL * uxL + uyL * uyL) + pL / rhoL) +
+                         (R)0.5 * rhoFlux * (CvR * TR + (R)0.5 * (uxR * uxR + uyR * uyR) + pR / rhoR);
+            if (isShock)
+                kfvs_flux<R>(rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Rg,
+                             CvR, magSf, Sfx, Sfy);
+            else
+                roe_flux<R>(rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, dP,
+                            dRho, (R)((Cv + Rgas) / Cv), CvR, magSf, Sfx, Sfy);
+            out0 = rhoFlux;
+            out1 = rhoUxFlux;
+            out2 = rhoUyFlux;
+            out3 = rhoEFlux;
+        };
+        if (p.ngp == 0) {
+            R rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux;
+            point_flux(dxL, dyL, dxR, dyR, p.fv[2 * f], p.fv[2 * f + 1], rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
+            p.flux[f] = double4{(double)rhoFlux, (double)rhoUxFlux, (double)rhoUyFlux, (double)rhoEFlux};
+        } else {
+            // extension: weighted sum over the face's quadrature points
+            double4 acc = double4{0.0, 0.0, 0.0, 0.0};
+            for (int g = 0; g < p.ngp; ++g) {
+                const double4 d = p.fgd[(size_t)g * nf + f];
+                R rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux;
+                point_flux((R)d.x, (R)d.y, (R)d.z, (R)d.w, p.fvg[((size_t)g * nf + f) * 2],
+                           p.fvg[((size_t)g * nf + f) * 2 + 1], rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
+                const double w = p.gpw[g];
+                if (g == 0) {
+                    acc = double4{w * (double)rhoFlux, w * (double)rhoUxFlux, w * (double)rhoUyFlux,
+                                  w * (double)rhoEFlux};
+                } else {
+                    acc.x += w * (double)rhoFlux;
+                    acc.y += w * (double)rhoUxFlux;
+                    acc.z += w * (double)rhoUyFlux;
+                    acc.w += w * (double)rhoEFlux;
+                }
+            }
+            p.flux[f] = acc;
+        }
     } else if (i < nf + p.nb) {
         // boundary faces: physical flux of the ghost state (cudaRiemannSolver.h:254-270)
         const int b = i - nf;

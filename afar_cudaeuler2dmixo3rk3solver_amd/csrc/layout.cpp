@@ -105,6 +105,24 @@ RefArrays make_ref_arrays(const Mesh& m, const Precomputed& pc, const double* W0
         ra.FSHIFT[2 * f] = m.fshift[f].x;
         ra.FSHIFT[2 * f + 1] = m.fshift[f].y;
     }
+    if (pc.ngp) {
+        const int ngp = pc.ngp;
+        ra.ngp = ngp;
+        ra.GPW = pc.gp_w;
+        ra.RBFgp = pc.rbf_gp;
+        ra.GPD.resize(4 * (size_t)ngp * nf);
+        for (int g = 0; g < ngp; ++g)
+            for (int f = 0; f < nf; ++f) {
+                const Vec2 x = face_gauss_point(m, f, pc.gp_xi[g]);
+                const int own = m.own[f], nei = m.nei[f];
+                const double xn = m.C[nei].x + m.fshift[f].x, yn = m.C[nei].y + m.fshift[f].y;
+                double* d = &ra.GPD[4 * ((size_t)g * nf + f)];
+                d[0] = x.x - m.C[own].x;
+                d[1] = x.y - m.C[own].y;
+                d[2] = x.x - xn;
+                d[3] = x.y - yn;
+            }
+    }
     apply_bc_host(ra);
     conserved_from_primitive(ra, R, Cv);
     return ra;
@@ -238,6 +256,17 @@ RefArrays relabel(const RefArrays& ra, const Relabel& r) {
         if (!ra.RBFbasis.empty())
             std::copy(ra.RBFbasis.begin() + W * f, ra.RBFbasis.begin() + W * (f + 1), o.RBFbasis.begin() + W * g);
     }
+    o.ngp = ra.ngp;
+    o.GPW = ra.GPW;
+    o.GPD.resize(ra.GPD.size());
+    o.RBFgp.resize(ra.RBFgp.size());
+    for (int g = 0; g < ra.ngp; ++g)
+        for (int h = 0; h < nf; ++h) {
+            const int f = r.face_old[h];
+            for (int k = 0; k < 4; ++k) o.GPD[4 * ((size_t)g * nf + h) + k] = ra.GPD[4 * ((size_t)g * nf + f) + k];
+            std::copy(ra.RBFgp.begin() + W * ((size_t)g * nf + f), ra.RBFgp.begin() + W * ((size_t)g * nf + f + 1),
+                      o.RBFgp.begin() + W * ((size_t)g * nf + h));
+        }
     o.boundaryFacesNum = ra.boundaryFacesNum;
     o.boundaryFaceNeiLabel.resize(nb);
     for (int b = 0; b < nb; ++b) o.boundaryFaceNeiLabel[b] = cell(ra.boundaryFaceNeiLabel[b]);
@@ -266,7 +295,11 @@ RefArrays union_ref_arrays(const std::vector<const RefArrays*>& parts, BatchInde
         u.maxS = std::max(u.maxS, a.maxS);
         u.maxC = std::max(u.maxC, a.maxC);
         u.npatch += a.npatch;
+        if (a.ngp != parts[0]->ngp || a.GPW != parts[0]->GPW)
+            throw Error(1, "every mesh of a batch needs the same face quadrature");
     }
+    u.ngp = parts[0]->ngp;
+    u.GPW = parts[0]->GPW;
     u.nc = (int)nc;
     u.nf = (int)nf;
     u.nb = (int)nb;
@@ -286,6 +319,8 @@ RefArrays union_ref_arrays(const std::vector<const RefArrays*>& parts, BatchInde
     u.RBFbasis.assign(2 * (size_t)(maxS + 1) * nf, 0.0);
     u.FSHIFT.resize(2 * (size_t)nf);
     u.BVALUE.resize(4 * (size_t)nb);
+    u.GPD.assign(4 * (size_t)u.ngp * nf, 0.0);
+    u.RBFgp.assign((size_t)u.ngp * 2 * (maxS + 1) * nf, 0.0);
     for (int k = 0; k < n; ++k) {
         const RefArrays& a = *parts[k];
         const int co = idx.cell_off[k], fo = idx.face_off[k], bo = idx.bface_off[k];
@@ -328,6 +363,14 @@ RefArrays union_ref_arrays(const std::vector<const RefArrays*>& parts, BatchInde
                 for (int i = 0; i <= a.maxS; ++i)
                     u.RBFbasis[(2 * d + side) * (size_t)(maxS + 1) + i] =
                         a.RBFbasis[(2 * (size_t)f + side) * (size_t)(a.maxS + 1) + i];
+            for (int g = 0; g < u.ngp; ++g) {
+                for (int q = 0; q < 4; ++q)
+                    u.GPD[4 * ((size_t)g * nf + d) + q] = a.GPD[4 * ((size_t)g * a.nf + f) + q];
+                for (int side = 0; side < 2; ++side)
+                    for (int i = 0; i <= a.maxS; ++i)
+                        u.RBFgp[(((size_t)g * nf + d) * 2 + side) * (size_t)(maxS + 1) + i] =
+                            a.RBFgp[(((size_t)g * a.nf + f) * 2 + side) * (size_t)(a.maxS + 1) + i];
+            }
         }
         for (int b = 0; b < a.nb; ++b) {
             const size_t d = bo + b;
@@ -425,6 +468,10 @@ DevLayout make_dev_layout(const RefArrays& ra, int tile) {
     L.slot_d.assign(2 * nslot, 0.0);
     L.slot_e.assign(2 * nslot, 0.0);
     L.rbf.assign((size_t)L.nwaves * KR * 64, 0.0);
+    L.ngp = ra.ngp;
+    L.gpw = ra.GPW;
+    L.fgd = ra.GPD;
+    L.rbf_gp.assign((size_t)L.nwaves * ra.ngp * KR * 64, 0.0);
     L.vol = ra.cellVolume;
     const double* CELL = ra.CELL.data();
     const double* FACE = ra.FACE.data();
@@ -499,6 +546,12 @@ DevLayout make_dev_layout(const RefArrays& ra, int tile) {
                     const int side = sf < 0;
                     const double* a = &ra.RBFbasis[(size_t)2 * (maxS + 1) * f + (size_t)side * (maxS + 1)];
                     for (int i = 0; i <= ra.stencilSize[c]; ++i) L.rbf[L.sell(c, fi * (maxS + 1) + i, KR)] = a[i];
+                    for (int g = 0; g < ra.ngp; ++g) {
+                        const double* ag =
+                            &ra.RBFgp[(((size_t)g * nf + f) * 2 + side) * (size_t)(maxS + 1)];
+                        for (int i = 0; i <= ra.stencilSize[c]; ++i)
+                            L.rbf_gp[L.sell(c, (g * maxC + fi) * (maxS + 1) + i, ra.ngp * KR)] = ag[i];
+                    }
                 } else {
                     const int b = f - nf;
                     d[0] = F[0] - CELL[2 * c];

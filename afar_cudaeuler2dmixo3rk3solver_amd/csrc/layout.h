@@ -38,6 +38,11 @@ struct RefArrays {
     std::vector<int32_t> cellFaces;  // maxC*nc
     std::vector<double> FSHIFT;      // 2*nf: neighbour centre seen from the owner = CELL[nei] + FSHIFT
     std::vector<double> BVALUE;      // 4*nb: fixed values for label 6
+    // extension (faceQuadrature): ngp flux points per internal face
+    int ngp = 0;
+    std::vector<double> GPW;         // ngp weights, sum 1
+    std::vector<double> GPD;         // 4*ngp*nf: per (g, f) point - owner centre, point - neighbour centre
+    std::vector<double> RBFgp;       // ngp*2*(maxS+1)*nf: RBFbasis at each point
 };
 
 // Builds the reference arrays of a mesh.  W0 holds p, Ux, Uy, T per cell in
@@ -69,6 +74,11 @@ struct DevLayout {
     std::vector<double> bgeo;             // 4*nb: Sfx, Sfy, magSf, 0
     std::vector<double> bval;             // 4*nb
     std::vector<double> vol;              // nc
+    // extension (faceQuadrature): flux points
+    int ngp = 0;
+    std::vector<double> gpw;              // ngp
+    std::vector<double> rbf_gp;           // per slot and point: element (g*maxC + k)*(maxS+1) + i, width 64
+    std::vector<double> fgd;              // 4*ngp*nf as RefArrays::GPD
     size_t sell(int c, int k, int K) const { return ((size_t)(c >> 6) * K + k) * 64 + (c & 63); }
 };
 

@@ -324,6 +324,13 @@ void gauss_cell(const Mesh& m, int cell, int cOrder, std::vector<Vec2>& pts, std
     }
 }
 
+void face_gauss_rule(int fOrder, std::vector<double>& xi, std::vector<double>& w) { bar_rule(fOrder, xi, w); }
+
+Vec2 face_gauss_point(const Mesh& m, int f, double xi) {
+    const Vec2 p0 = m.fp0[f], p1 = m.fp1[f];
+    return Vec2{0.5 * p0.x * (1 - xi) + 0.5 * p1.x * (1 + xi), 0.5 * p0.y * (1 - xi) + 0.5 * p1.y * (1 + xi)};
+}
+
 // Boundary-face Gauss points; weights include the face height and are then
 // divided by |S_f| so that they average over the face
 // (d2GaussQuadrature.C:217-279 with basicCUDAsolver.cpp:446-453).
@@ -389,6 +396,14 @@ Precomputed precompute(const Mesh& m, const Schemes& sc) {
     pc.rbf.assign((size_t)2 * W * nf, 0.0);
     pc.trouble.assign(nc, 0);
     pc.refDistance.assign(nc, 0.0);
+    if (sc.faceQuadrature) {
+        std::vector<double> sw;
+        face_gauss_rule(sc.fOrder, pc.gp_xi, sw);
+        pc.ngp = (int)pc.gp_xi.size();
+        pc.gp_w.resize(pc.ngp);
+        for (int g = 0; g < pc.ngp; ++g) pc.gp_w[g] = 0.5 * sw[g];  // the bar rule's weights sum to 2
+        pc.rbf_gp.assign((size_t)pc.ngp * 2 * W * nf, 0.0);
+    }
     if (!sc.recalcBasis) return pc;  // coefficients come from read_basis_lists
     const double sf = sc.shapeFactor;
     std::vector<double> tp, tw;
@@ -433,6 +448,11 @@ Precomputed precompute(const Mesh& m, const Schemes& sc) {
                 if (trouble) {
                     out[0] = 1.0;
                     for (int i = 1; i < n; ++i) out[i] = 0.0;
+                    for (int g = 0; g < pc.ngp; ++g) {
+                        double* og = &pc.rbf_gp[((size_t)g * nf + f) * 2 * W + (size_t)side * W];
+                        og[0] = 1.0;
+                        for (int i = 1; i < n; ++i) og[i] = 0.0;
+                    }
                     continue;
                 }
                 // face centre in c's frame
@@ -440,6 +460,14 @@ Precomputed precompute(const Mesh& m, const Schemes& sc) {
                 if (side) xf = Vec2{xf.x - m.fshift[f].x, xf.y - m.fshift[f].y};
                 rbf_face(n, G, X, xf.x, xf.y, ref, sf, b, a);
                 for (int i = 0; i < n; ++i) out[i] = a[i];
+                // flux quadrature points in c's frame (extension)
+                for (int g = 0; g < pc.ngp; ++g) {
+                    Vec2 xg = face_gauss_point(m, f, pc.gp_xi[g]);
+                    if (side) xg = Vec2{xg.x - m.fshift[f].x, xg.y - m.fshift[f].y};
+                    rbf_face(n, G, X, xg.x, xg.y, ref, sf, b, a);
+                    double* og = &pc.rbf_gp[((size_t)g * nf + f) * 2 * W + (size_t)side * W];
+                    for (int i = 0; i < n; ++i) og[i] = a[i];
+                }
             }
         }
     });

@@ -34,6 +34,13 @@ struct RbfArgs {
     double* rbf;                // 2 * W * nf
     uint8_t* trouble;           // nc
     double* ref;                // nc
+    // flux quadrature points (extension): ngp points at xi[g] of each face's
+    // end points fp0, fp1 (owner frame); rbf_gp: ngp * 2 * W * nf
+    int ngp;
+    double xi[4];
+    const double2* fp0;
+    const double2* fp1;
+    double* rbf_gp;
 };
 
 // Few waves per SIMD: each thread holds about 11 KB of private work arrays.
@@ -85,6 +92,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
         if (trouble) {
             o[0] = 1.0;
             for (int i = 1; i < n; ++i) o[i] = 0.0;
+            for (int g = 0; g < a.ngp; ++g) {
+                double* og = a.rbf_gp + ((size_t)g * a.nf + f) * 2 * a.W + (size_t)side * a.W;
+                og[0] = 1.0;
+                for (int i = 1; i < n; ++i) og[i] = 0.0;
+            }
             continue;
         }
         double xf = a.Cf[f].x, yf = a.Cf[f].y;
@@ -94,6 +106,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
         }
         rbf_face(n, G, X, xf, yf, ref, a.sf, b, out);
         for (int i = 0; i < n; ++i) o[i] = out[i];
+        for (int g = 0; g < a.ngp; ++g) {  // the host's face_gauss_point, same operations
+            const double t = a.xi[g];
+            double xg = 0.5 * a.fp0[f].x * (1 - t) + 0.5 * a.fp1[f].x * (1 + t);
+            double yg = 0.5 * a.fp0[f].y * (1 - t) + 0.5 * a.fp1[f].y * (1 + t);
+            if (side) {
+                xg = xg - a.fshift[f].x;
+                yg = yg - a.fshift[f].y;
+            }
+            rbf_face(n, G, X, xg, yg, ref, a.sf, b, out);
+            double* og = a.rbf_gp + ((size_t)g * a.nf + f) * 2 * a.W + (size_t)side * a.W;
+            for (int i = 0; i < n; ++i) og[i] = out[i];
+        }
     }
 }
 
@@ -121,14 +145,17 @@ void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
     std::vector<double> tp, tw;
     std::vector<int64_t> tptr;
     gauss_tables(m, sc, tp, tw, tptr);
-    std::vector<double2> C(nc + nb), Cf(nf), fsh(nf), shift((size_t)nc * kMaxValidStencil);
+    std::vector<double2> C(nc + nb), Cf(nf), fsh(nf), shift((size_t)nc * kMaxValidStencil), p0(nf), p1(nf);
     for (int i = 0; i < nc + nb; ++i) C[i] = double2{m.C[i].x, m.C[i].y};
     for (int f = 0; f < nf; ++f) {
         Cf[f] = double2{m.Cf[f].x, m.Cf[f].y};
         fsh[f] = double2{m.fshift[f].x, m.fshift[f].y};
+        p0[f] = double2{m.fp0[f].x, m.fp0[f].y};
+        p1[f] = double2{m.fp1[f].x, m.fp1[f].y};
     }
+    if (pc.ngp > 4) throw Error(3, "more than four flux quadrature points");
     for (size_t i = 0; i < shift.size(); ++i) shift[i] = double2{st.sten_shift[i].x, st.sten_shift[i].y};
-    Buf bS, bCc, bSten, bShift, bCF, bC, bTp, bTw, bTptr, bCf, bFsh, bRbf, bTr, bRef;
+    Buf bS, bCc, bSten, bShift, bCF, bC, bTp, bTw, bTptr, bCf, bFsh, bRbf, bTr, bRef, bP0, bP1, bGp;
     RbfArgs a{};
     a.nc = nc;
     a.nf = nf;
@@ -148,6 +175,11 @@ void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
     a.rbf = bRbf.put<double>(nullptr, (size_t)2 * W * nf);
     a.trouble = bTr.put<uint8_t>(nullptr, nc);
     a.ref = bRef.put<double>(nullptr, nc);
+    a.ngp = pc.ngp;
+    for (int g = 0; g < pc.ngp; ++g) a.xi[g] = pc.gp_xi[g];
+    a.fp0 = pc.ngp ? bP0.put(p0.data(), p0.size()) : nullptr;
+    a.fp1 = pc.ngp ? bP1.put(p1.data(), p1.size()) : nullptr;
+    a.rbf_gp = pc.ngp ? bGp.put<double>(nullptr, (size_t)pc.ngp * 2 * W * nf) : nullptr;
     // own stream: waits for this work only, not for other solvers on the device
     hipStream_t s = nullptr;
     check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
@@ -156,6 +188,8 @@ void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
         ~StreamGuard() { (void)hipStreamDestroy(s); }
     } guard{s};
     check_hip(hipMemsetAsync(a.rbf, 0, sizeof(double) * 2 * W * (size_t)nf, s), "hipMemsetAsync");
+    if (pc.ngp)
+        check_hip(hipMemsetAsync(a.rbf_gp, 0, sizeof(double) * pc.ngp * 2 * W * (size_t)nf, s), "hipMemsetAsync");
     hipLaunchKernelGGL(k_rbf_precompute, dim3((nc + 63) / 64), dim3(64), 0, s, a);
     check_hip(hipGetLastError(), "k_rbf_precompute");
     pc.rbf.assign((size_t)2 * W * nf, 0.0);
@@ -166,6 +200,11 @@ void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
     check_hip(hipMemcpyAsync(pc.trouble.data(), a.trouble, nc, hipMemcpyDeviceToHost, s), "download");
     check_hip(hipMemcpyAsync(pc.refDistance.data(), a.ref, sizeof(double) * nc, hipMemcpyDeviceToHost, s),
               "download");
+    pc.rbf_gp.assign((size_t)pc.ngp * 2 * W * nf, 0.0);
+    if (pc.ngp)
+        check_hip(hipMemcpyAsync(pc.rbf_gp.data(), a.rbf_gp, sizeof(double) * pc.rbf_gp.size(), hipMemcpyDeviceToHost,
+                                 s),
+                  "download");
     check_hip(hipStreamSynchronize(s), "k_rbf_precompute");
     int t = 0;
     for (uint8_t v : pc.trouble) t += v;

@@ -64,6 +64,7 @@ class SolverImpl final : public DeviceSolver {
         if ((cfg.reconLanes == 4 || cfg.reconLanes == 2) && L_.maxC > 4)
             throw Error(1, "reconLanes 2 and 4 need cells with at most four faces");
         if (cfg.reconLanes == 2 && L_.tile > 128) throw Error(1, "reconLanes 2 needs a tile of at most 128 cells");
+        if (cfg.reconLanes == 2 && ra.ngp) throw Error(1, "reconLanes 2 does not support faceQuadrature");
         if (cfg.reconLanes != 0)
             lanes_ = cfg.reconLanes;
         else
@@ -342,6 +343,7 @@ class SolverImpl final : public DeviceSolver {
         pick("smooth", P_.smooth, nc + nb);
         pick("ftype", P_.ftype, nc);
         pick("fv", P_.fv, sizeof(v4) * 2 * nf);
+        pick("fvg", P_.fvg, sizeof(v4) * 2 * nf * L_.ngp);
         pick("flux", P_.flux, sizeof(double4) * (nf + nb));
         pick("lsq", P_.lsq, sizeof(v2) * (size_t)L_.nwaves * L_.maxC * 64);
         if (!src) throw Error(1, std::string("unknown device array ") + what);
@@ -482,6 +484,17 @@ class SolverImpl final : public DeviceSolver {
         P_.fv = m_fv.as<v4>();
         m_fvT.alloc(sizeof(R) * 2 * nf);
         P_.fvT = m_fvT.as<R>();
+        P_.ngp = L_.ngp;
+        if (L_.ngp > 4) throw Error(3, "more than four flux quadrature points");
+        for (int g = 0; g < L_.ngp; ++g) P_.gpw[g] = L_.gpw[g];
+        if (L_.ngp) {
+            std::vector<R> rg(L_.rbf_gp.size());
+            for (size_t i = 0; i < rg.size(); ++i) rg[i] = (R)L_.rbf_gp[i];
+            P_.rbf_gp = put(m_rbf_gp, rg.data(), rg.size());
+            P_.fgd = reinterpret_cast<const double4*>(put(m_fgd, L_.fgd.data(), L_.fgd.size()));
+            m_fvg.alloc(sizeof(v4) * 2 * nf * L_.ngp);
+            P_.fvg = m_fvg.as<v4>();
+        }
         m_flux.alloc(sizeof(double4) * (nf + nb));
         P_.flux = m_flux.as<double4>();
         P_.bcell = put(m_bcell, L_.bcell.data(), nb);
@@ -541,7 +554,7 @@ class SolverImpl final : public DeviceSolver {
     int lanes_ = 1;  // threads per cell of the reconstruction kernel
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fv, m_flux, m_bcell,
-        m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res;
+        m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res, m_rbf_gp, m_fgd, m_fvg;
 };
 
 // A solver on relabelled cells and faces (compact_tile_order) that reads and
@@ -644,7 +657,13 @@ class RelabeledSolver final : public DeviceSolver {
             faces(nf_, 2);
         else if (w == "flux")
             faces(nf_ + nb_, 1);
-        else
+        else if (w == "fvg" && nf_ > 0) {  // ngp blocks of nf faces, two states each
+            const size_t rec = 2 * 4 * (cfg_.precision == 1 ? sizeof(float) : sizeof(double));
+            const size_t blocks = bytes / ((size_t)nf_ * rec);
+            for (size_t g = 0; g < blocks; ++g)
+                for (int i = 0; i < nf_; ++i)
+                    memcpy(o + rec * (g * nf_ + i), b.data() + rec * (g * nf_ + r_.face_new[i]), rec);
+        } else
             memcpy(out, b.data(), bytes);
     }
     void run_stage_kernels(int upto, double dt, int stage) override { in_->run_stage_kernels(upto, dt, stage); }

@@ -19,9 +19,9 @@ STENCIL_TYPES = {"faceNeighbour": 0, "edgeNeighbour": 1}
 
 def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilType="edgeNeighbour",
             recalcBasis=True, cOrder=5, fOrder=4, precision="fp64", exactRK=False, tile=64,
-            boundaryFirstOrder=False, reconLanes=0, cellOrder=0):
+            boundaryFirstOrder=False, reconLanes=0, cellOrder=0, faceQuadrature=0):
     """afar_schemes with the reference's key names and defaults (the last
-    six are this library's extensions)."""
+    seven are this library's extensions)."""
     s = Schemes()
     lib().afar_schemes_default(C.byref(s))
     s.R, s.Cv, s.maxCo, s.shapeFactor = R, Cv, maxCo, shapeFactor
@@ -34,6 +34,7 @@ def schemes(R=0.7142857, Cv=1.7857142857, maxCo=0.5, shapeFactor=10.0, stencilTy
     s.boundaryFirstOrder = int(bool(boundaryFirstOrder))
     s.reconLanes = int(reconLanes)
     s.cellOrder = int(cellOrder)
+    s.faceQuadrature = int(faceQuadrature)
     return s
 
 
@@ -210,6 +211,22 @@ class Mesh:
         check(lib().afar_mesh_rbf_basis(self._h, dptr(rbf), ptr(tr, C.c_uint8), dptr(ref)))
         return rbf, tr, ref
 
+    def face_quadrature(self):
+        """Flux quadrature of a mesh precomputed with faceQuadrature 1, or
+        None: dict(xi (ngp,), weights (ngp,), offsets (ngp, nf, 4): point -
+        owner centre and point - neighbour centre, rbf (ngp, nf, 2, maxS+1))."""
+        s = self.sizes()
+        n = C.c_int32()
+        check(lib().afar_mesh_face_quadrature(self._h, C.byref(n), None, None, None, None))
+        ngp = n.value
+        if ngp == 0:
+            return None
+        q = dict(xi=np.zeros(ngp), weights=np.zeros(ngp), offsets=np.zeros((ngp, s["nf"], 4)),
+                 rbf=np.zeros((ngp, s["nf"], 2, s["maxS"] + 1)))
+        check(lib().afar_mesh_face_quadrature(self._h, C.byref(n), dptr(q["xi"]), dptr(q["weights"]),
+                                              dptr(q["offsets"]), dptr(q["rbf"])))
+        return q
+
     def gauss_points(self, member, order):
         xy = np.zeros((32, 2))
         w = np.zeros(32)
@@ -242,6 +259,9 @@ class Mesh:
         fp = field_pointer(a)
         w0 = np.ascontiguousarray(W0, dtype=np.float64)
         check(lib().afar_mesh_export_reference(self._h, C.byref(sc), dptr(w0), C.byref(fp)))
+        q = self.face_quadrature()
+        if q is not None:  # extension: flux quadrature points
+            a["gpWeights"], a["gpOffsets"], a["RBFgp"] = q["weights"], q["offsets"], q["rbf"]
         a["sizes"] = s
         return a
 

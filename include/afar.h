@@ -68,6 +68,13 @@ typedef struct afar_schemes {
     int32_t cellOrder;   /* device order of cells, same results: 0 compact tiles (cells along a
                             Hilbert curve through their centres, faces by owner), 1 the mesh's
                             order; every array the library returns is in the mesh's order */
+    int32_t faceQuadrature; /* extension, 0 = reference: the flux of an internal face is evaluated
+                            at its centre (cudaRiemannSolver.h:178-182).  1: at the points of the
+                            fOrder bar rule (d2GaussQuadrature.C:153-214), weights summing to
+                            one; face states there come from per-point RBF coefficients (O3) or
+                            the limited linear reconstruction (TVD) with the limiters, flags and
+                            switch of the centre.  Set it for the mesh precompute and the solver
+                            alike; fOrder 1 reproduces 0 bit for bit */
 } afar_schemes;
 
 const char* afar_last_error(void);
@@ -159,6 +166,13 @@ int afar_mesh_precompute_gpu(afar_mesh* m, const afar_schemes* s);
 int afar_mesh_stencils(const afar_mesh* m, uint8_t* S, uint8_t* C, int32_t* stencil, int32_t* cellFaces);
 /* rbf 2*(maxS+1)*nf (RBFbasis layout), trouble nc, refDistance nc; NULL to skip. */
 int afar_mesh_rbf_basis(const afar_mesh* m, double* rbf, uint8_t* trouble, double* refDistance);
+/* Flux quadrature of a mesh precomputed with faceQuadrature 1 (extension, no
+ * reference counterpart; *ngp = 0 otherwise): xi ngp points on [-1, 1],
+ * weights ngp (sum 1), offsets 4*ngp*nf (per point g and internal face f:
+ * point - owner centre, point - neighbour centre), rbf ngp*2*(maxS+1)*nf
+ * (RBFbasis layout per point).  NULL to skip an output. */
+int afar_mesh_face_quadrature(const afar_mesh* m, int32_t* ngp, double* xi, double* weights, double* offsets,
+                              double* rbf);
 /* Gauss points of a stencil member (cell < nc: cell rule cOrder; pseudo-cell: face rule fOrder). */
 int afar_mesh_gauss_points(const afar_mesh* m, int32_t member, int32_t order, double* xy, double* w, int32_t cap,
                            int32_t* count);

@@ -23,6 +23,10 @@
 // Periodic meshes: the neighbour centre seen from the owner is CELL[nei] +
 // FSHIFT[f] (zero for ordinary meshes, where every expression reduces to the
 // reference's).  Built with -ffp-contract=off (see Makefile).
+// Library extensions restated here as the checker of the device path:
+// boundaryFirstOrder (oracle_bvd) and faceQuadrature (ngp > 0: face states
+// and fluxes at the faces' quadrature points, oracle_reconstruct and
+// oracle_evaluate_flux); with ngp = 0 every function is the reference's.
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
@@ -60,6 +64,12 @@ typedef struct oracle_fields {
     double* Flux;                    // 4*(nf+nb)
     double* Res;                     // 4*nc
     int32_t boundaryFirstOrder;      // library extension (0 = reference behaviour)
+    // library extension (faceQuadrature): flux at ngp points per internal face
+    int32_t ngp;                     // 0 = face centre (reference)
+    const double* gpWeights;         // ngp, sum 1
+    const double* gpOffsets;         // 4*ngp*nf: point - owner centre, point - neighbour centre
+    const double* RBFgp;             // ngp*2*(maxS+1)*nf
+    double* faceGP;                  // 8*ngp*nf: [(g*nf + f)*2 + side] * 4
 } oracle_fields;
 
 }  // extern "C"
@@ -338,6 +348,15 @@ void oracle_reconstruct(oracle_fields* f) {
                 o[1] = Wc[1] + g[GUXX] * s.dx + g[GUYX] * s.dy;
                 o[2] = Wc[2] + g[GUXY] * s.dx + g[GUYY] * s.dy;
                 o[3] = Wc[3] + g[GTX] * s.dx + g[GTY] * s.dy;
+                for (int q = 0; q < f->ngp; ++q) {  // the same linear state at the flux points
+                    const double* d = &f->gpOffsets[4 * ((size_t)q * f->nf + s.face)];
+                    const double gx = s.isNei ? d[2] : d[0], gy = s.isNei ? d[3] : d[1];
+                    double* og = &f->faceGP[4 * (((size_t)q * f->nf + s.face) * 2 + s.isNei)];
+                    og[0] = Wc[0] + g[GPX] * gx + g[GPY] * gy;
+                    og[1] = Wc[1] + g[GUXX] * gx + g[GUYX] * gy;
+                    og[2] = Wc[2] + g[GUXY] * gx + g[GUYY] * gy;
+                    og[3] = Wc[3] + g[GTX] * gx + g[GTY] * gy;
+                }
             }
             f->shockIndicator[2 * (size_t)celli] = 0;
             continue;
@@ -376,6 +395,28 @@ void oracle_reconstruct(oracle_fields* f) {
             limiter_process(tmpT, phif, d1, d2, yy, Tc, TMax, TMin, g[GTX], g[GTY], dx, dy);
             o[3] = (fabs(TMax - TMin) >= SMALL) * Tf + (fabs(TMax - TMin) < SMALL) * phif;
             isNotTrouble = phif > 0.0;
+            // flux points: RBF sums with each point's coefficients, or the
+            // unlimited linear state where the compact range is flat
+            for (int q = 0; q < f->ngp; ++q) {
+                const double* ag = &f->RBFgp[(((size_t)q * f->nf + s.face) * 2 + s.isNei) * (size_t)(maxS + 1)];
+                double pg = ag[0] * pc, Uxg = ag[0] * Uxc, Uyg = ag[0] * Uyc, Tg = ag[0] * Tc;
+                for (int ci = 0; ci < S; ++ci) {
+                    const double* Wj = &f->FD[4 * (size_t)st[ci]];
+                    pg += ag[ci + 1] * Wj[0];
+                    Uxg += ag[ci + 1] * Wj[1];
+                    Uyg += ag[ci + 1] * Wj[2];
+                    Tg += ag[ci + 1] * Wj[3];
+                }
+                const double* d = &f->gpOffsets[4 * ((size_t)q * f->nf + s.face)];
+                const double gx = s.isNei ? d[2] : d[0], gy = s.isNei ? d[3] : d[1];
+                double* og = &f->faceGP[4 * (((size_t)q * f->nf + s.face) * 2 + s.isNei)];
+                og[0] = (fabs(pMax - pMin) >= SMALL) * pg + (fabs(pMax - pMin) < SMALL) * (pc + (g[GPX] * gx + g[GPY] * gy));
+                og[1] = (fabs(UxMax - UxMin) >= SMALL) * Uxg +
+                        (fabs(UxMax - UxMin) < SMALL) * (Uxc + (g[GUXX] * gx + g[GUYX] * gy));
+                og[2] = (fabs(UyMax - UyMin) >= SMALL) * Uyg +
+                        (fabs(UyMax - UyMin) < SMALL) * (Uyc + (g[GUXY] * gx + g[GUYY] * gy));
+                og[3] = (fabs(TMax - TMin) >= SMALL) * Tg + (fabs(TMax - TMin) < SMALL) * (Tc + (g[GTX] * gx + g[GTY] * gy));
+            }
         }
         lim[0] = fmax(tmpP, 0.0) * isNotTrouble;
         lim[1] = fmax(tmpUx, 0.0) * isNotTrouble;
@@ -440,61 +481,79 @@ void oracle_bvd(oracle_fields* f) {
     }
 }
 
+// The flux of face faceI from the states reconstructed at a point with
+// offsets (dxL, dyL) from the owner and (dxR, dyR) from the neighbour; fo, fn
+// are the high-order states there (cudaRiemannSolver.h:155-251).
+static void point_flux(const oracle_fields* f, int faceI, double R, double Cv, double deltaRxLeft, double deltaRyLeft,
+                       double deltaRxRight, double deltaRyRight, const double* fo, const double* fn, double* out) {
+    const int own = f->neighbour[2 * faceI];
+    const int nei = f->neighbour[2 * faceI + 1];
+    const double* F = &f->FACE[5 * (size_t)faceI];
+    const bool isTVD_left = f->shockIndicator[2 * (size_t)own + 1] != 0;
+    const bool isTVD_right = f->shockIndicator[2 * (size_t)nei + 1] != 0;
+    const bool isShock = f->shockIndicator[2 * (size_t)own + 1] < 0 || f->shockIndicator[2 * (size_t)nei + 1] < 0;
+    const double magSf = F[2], Sfx = F[3], Sfy = F[4];
+    const double* Wo = &f->FD[4 * (size_t)own];
+    const double* Wn = &f->FD[4 * (size_t)nei];
+    const double* Lo = &f->limiter[4 * (size_t)own];
+    const double* Ln = &f->limiter[4 * (size_t)nei];
+    const double* Go = &f->gradFD[8 * (size_t)own];
+    const double* Gn = &f->gradFD[8 * (size_t)nei];
+    const double pLeftTVD = Wo[0] + Lo[0] * (Go[GPX] * deltaRxLeft + Go[GPY] * deltaRyLeft);
+    const double UxLeftTVD = Wo[1] + Lo[1] * (Go[GUXX] * deltaRxLeft + Go[GUYX] * deltaRyLeft);
+    const double UyLeftTVD = Wo[2] + Lo[2] * (Go[GUXY] * deltaRxLeft + Go[GUYY] * deltaRyLeft);
+    const double TLeftTVD = Wo[3] + Lo[3] * (Go[GTX] * deltaRxLeft + Go[GTY] * deltaRyLeft);
+    const double rhoLeftTVD = pLeftTVD / (R * TLeftTVD);
+    const double pRightTVD = Wn[0] + Ln[0] * (Gn[GPX] * deltaRxRight + Gn[GPY] * deltaRyRight);
+    const double UxRightTVD = Wn[1] + Ln[1] * (Gn[GUXX] * deltaRxRight + Gn[GUYX] * deltaRyRight);
+    const double UyRightTVD = Wn[2] + Ln[2] * (Gn[GUXY] * deltaRxRight + Gn[GUYY] * deltaRyRight);
+    const double TRightTVD = Wn[3] + Ln[3] * (Gn[GTX] * deltaRxRight + Gn[GTY] * deltaRyRight);
+    const double rhoRightTVD = pRightTVD / (R * TRightTVD);
+    const double pLeft = (1 - isTVD_left) * fo[0] + isTVD_left * pLeftTVD;
+    const double UxLeft = (1 - isTVD_left) * fo[1] + isTVD_left * UxLeftTVD;
+    const double UyLeft = (1 - isTVD_left) * fo[2] + isTVD_left * UyLeftTVD;
+    const double TLeft = (1 - isTVD_left) * fo[3] + isTVD_left * TLeftTVD;
+    const double rhoLeft = pLeft / (R * TLeft);
+    const double pRight = (1 - isTVD_right) * fn[0] + isTVD_right * pRightTVD;
+    const double UxRight = (1 - isTVD_right) * fn[1] + isTVD_right * UxRightTVD;
+    const double UyRight = (1 - isTVD_right) * fn[2] + isTVD_right * UyRightTVD;
+    const double TRight = (1 - isTVD_right) * fn[3] + isTVD_right * TRightTVD;
+    const double rhoRight = pRight / (R * TRight);
+    double deltaP, deltaRho;
+    minimize_delta(deltaP, pLeft, pRight, pLeftTVD, pRightTVD);
+    minimize_delta(deltaRho, rhoLeft, rhoRight, rhoLeftTVD, rhoRightTVD);
+    oracle_central_flux(out, rhoLeft, rhoRight, pLeft, pRight, UxLeft, UxRight, UyLeft, UyRight, TLeft, TRight, Cv,
+                        Sfx, Sfy);
+    if (isShock)
+        oracle_kfvs_flux(out, rhoLeft, rhoRight, pLeft, pRight, UxLeft, UxRight, UyLeft, UyRight, TLeft, TRight, R, Cv,
+                         magSf, Sfx, Sfy);
+    else
+        oracle_roe_flux(out, rhoLeft, rhoRight, pLeft, pRight, UxLeft, UxRight, UyLeft, UyRight, TLeft, TRight, deltaP,
+                        deltaRho, R, Cv, magSf, Sfx, Sfy);
+}
+
 void oracle_evaluate_flux(oracle_fields* f, double R, double Cv) {
 #pragma omp parallel for schedule(static)
     for (int faceI = 0; faceI < f->nf; ++faceI) {
         const int own = f->neighbour[2 * faceI];
         const int nei = f->neighbour[2 * faceI + 1];
         const double* F = &f->FACE[5 * (size_t)faceI];
-        const double xn = f->CELL[2 * nei] + shx(f, faceI), yn = f->CELL[2 * nei + 1] + shy(f, faceI);
-        const double deltaRxLeft = F[0] - f->CELL[2 * own];
-        const double deltaRyLeft = F[1] - f->CELL[2 * own + 1];
-        const double deltaRxRight = F[0] - xn;
-        const double deltaRyRight = F[1] - yn;
-        const bool isTVD_left = f->shockIndicator[2 * (size_t)own + 1] != 0;
-        const bool isTVD_right = f->shockIndicator[2 * (size_t)nei + 1] != 0;
-        const bool isShock = f->shockIndicator[2 * (size_t)own + 1] < 0 || f->shockIndicator[2 * (size_t)nei + 1] < 0;
-        const double magSf = F[2], Sfx = F[3], Sfy = F[4];
-        const double* Wo = &f->FD[4 * (size_t)own];
-        const double* Wn = &f->FD[4 * (size_t)nei];
-        const double* Lo = &f->limiter[4 * (size_t)own];
-        const double* Ln = &f->limiter[4 * (size_t)nei];
-        const double* Go = &f->gradFD[8 * (size_t)own];
-        const double* Gn = &f->gradFD[8 * (size_t)nei];
-        const double* fo = &f->faceFD[4 * (2 * (size_t)faceI)];
-        const double* fn = &f->faceFD[4 * (2 * (size_t)faceI + 1)];
-        const double pLeftTVD = Wo[0] + Lo[0] * (Go[GPX] * deltaRxLeft + Go[GPY] * deltaRyLeft);
-        const double UxLeftTVD = Wo[1] + Lo[1] * (Go[GUXX] * deltaRxLeft + Go[GUYX] * deltaRyLeft);
-        const double UyLeftTVD = Wo[2] + Lo[2] * (Go[GUXY] * deltaRxLeft + Go[GUYY] * deltaRyLeft);
-        const double TLeftTVD = Wo[3] + Lo[3] * (Go[GTX] * deltaRxLeft + Go[GTY] * deltaRyLeft);
-        const double rhoLeftTVD = pLeftTVD / (R * TLeftTVD);
-        const double pRightTVD = Wn[0] + Ln[0] * (Gn[GPX] * deltaRxRight + Gn[GPY] * deltaRyRight);
-        const double UxRightTVD = Wn[1] + Ln[1] * (Gn[GUXX] * deltaRxRight + Gn[GUYX] * deltaRyRight);
-        const double UyRightTVD = Wn[2] + Ln[2] * (Gn[GUXY] * deltaRxRight + Gn[GUYY] * deltaRyRight);
-        const double TRightTVD = Wn[3] + Ln[3] * (Gn[GTX] * deltaRxRight + Gn[GTY] * deltaRyRight);
-        const double rhoRightTVD = pRightTVD / (R * TRightTVD);
-        const double pLeft = (1 - isTVD_left) * fo[0] + isTVD_left * pLeftTVD;
-        const double UxLeft = (1 - isTVD_left) * fo[1] + isTVD_left * UxLeftTVD;
-        const double UyLeft = (1 - isTVD_left) * fo[2] + isTVD_left * UyLeftTVD;
-        const double TLeft = (1 - isTVD_left) * fo[3] + isTVD_left * TLeftTVD;
-        const double rhoLeft = pLeft / (R * TLeft);
-        const double pRight = (1 - isTVD_right) * fn[0] + isTVD_right * pRightTVD;
-        const double UxRight = (1 - isTVD_right) * fn[1] + isTVD_right * UxRightTVD;
-        const double UyRight = (1 - isTVD_right) * fn[2] + isTVD_right * UyRightTVD;
-        const double TRight = (1 - isTVD_right) * fn[3] + isTVD_right * TRightTVD;
-        const double rhoRight = pRight / (R * TRight);
-        double deltaP, deltaRho;
-        minimize_delta(deltaP, pLeft, pRight, pLeftTVD, pRightTVD);
-        minimize_delta(deltaRho, rhoLeft, rhoRight, rhoLeftTVD, rhoRightTVD);
         double* out = &f->Flux[4 * (size_t)faceI];
-        oracle_central_flux(out, rhoLeft, rhoRight, pLeft, pRight, UxLeft, UxRight, UyLeft, UyRight, TLeft, TRight, Cv,
-                            Sfx, Sfy);
-        if (isShock)
-            oracle_kfvs_flux(out, rhoLeft, rhoRight, pLeft, pRight, UxLeft, UxRight, UyLeft, UyRight, TLeft, TRight, R,
-                             Cv, magSf, Sfx, Sfy);
-        else
-            oracle_roe_flux(out, rhoLeft, rhoRight, pLeft, pRight, UxLeft, UxRight, UyLeft, UyRight, TLeft, TRight,
-                            deltaP, deltaRho, R, Cv, magSf, Sfx, Sfy);
+        if (f->ngp == 0) {
+            const double xn = f->CELL[2 * nei] + shx(f, faceI), yn = f->CELL[2 * nei + 1] + shy(f, faceI);
+            point_flux(f, faceI, R, Cv, F[0] - f->CELL[2 * own], F[1] - f->CELL[2 * own + 1], F[0] - xn, F[1] - yn,
+                       &f->faceFD[4 * (2 * (size_t)faceI)], &f->faceFD[4 * (2 * (size_t)faceI + 1)], out);
+            continue;
+        }
+        // extension: weighted sum over the face's quadrature points, in point order
+        for (int q = 0; q < f->ngp; ++q) {
+            const double* d = &f->gpOffsets[4 * ((size_t)q * f->nf + faceI)];
+            double fq[4];
+            point_flux(f, faceI, R, Cv, d[0], d[1], d[2], d[3], &f->faceGP[4 * (((size_t)q * f->nf + faceI) * 2)],
+                       &f->faceGP[4 * (((size_t)q * f->nf + faceI) * 2 + 1)], fq);
+            const double w = f->gpWeights[q];
+            for (int k = 0; k < 4; ++k) out[k] = q == 0 ? w * fq[k] : out[k] + w * fq[k];
+        }
     }
     // every boundary face (the reference only covers faceI < nf of its grid)
     for (int b = 0; b < f->nb; ++b) {

@@ -27,6 +27,8 @@ class OracleFields(C.Structure):
         ("matrix", P(C.c_double)), ("gradFD", P(C.c_double)), ("limiter", P(C.c_double)),
         ("shockIndicator", P(C.c_int8)), ("faceFD", P(C.c_double)), ("Flux", P(C.c_double)), ("Res", P(C.c_double)),
         ("boundaryFirstOrder", C.c_int32),
+        ("ngp", C.c_int32), ("gpWeights", P(C.c_double)), ("gpOffsets", P(C.c_double)), ("RBFgp", P(C.c_double)),
+        ("faceGP", P(C.c_double)),
     ]
 
 
@@ -109,6 +111,14 @@ class OracleState:
         f.BVALUE = _p(a["boundaryValue"])
         f.shockIndicator = _p(a["shockIndicator"], C.c_int8)
         f.boundaryFirstOrder = int(bool(boundary_first_order))
+        if "RBFgp" in a:  # flux quadrature points (library extension)
+            ngp = a["RBFgp"].shape[0]
+            a["faceGP"] = np.zeros((ngp, nf, 2, 4))
+            f.ngp = ngp
+            f.gpWeights = _p(a["gpWeights"])
+            f.gpOffsets = _p(a["gpOffsets"])
+            f.RBFgp = _p(a["RBFgp"])
+            f.faceGP = _p(a["faceGP"])
         self.f = f
         lib().oracle_least_square_matrix(C.byref(f))
 
