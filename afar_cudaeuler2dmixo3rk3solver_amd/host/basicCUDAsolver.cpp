@@ -196,6 +196,8 @@ afar_schemes readSchemes(const dictionary& fv, const dictionary& control) {
     s.exactRK = fv.lookupOrDefault("CUDA.exactRK", false);
     s.tile = (int32_t)fv.lookupOrDefault("CUDA.tile", 64L);
     s.boundaryFirstOrder = fv.lookupOrDefault("CUDA.boundaryFirstOrder", false);
+    s.faceQuadrature = fv.lookupOrDefault("CUDA.faceQuadrature", false);  // extension: flux at the fOrder points
+    s.cellOrder = fv.found("CUDA.cellOrder") && fv.lookupWord("CUDA.cellOrder") == "mesh" ? 1 : 0;
     return s;
 }
 

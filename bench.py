@@ -58,12 +58,15 @@ def mesh_stats(mesh, field=None, precision="fp64"):
                 meanS=float(S.mean()), meanC=float(Cc.mean()))
 
 
-def kernel_bytes(st, w=8):
+def kernel_bytes(st, w=8, ngp=0):
     """Compulsory bytes per launch of each kernel (every array element touched
     once; neighbour re-reads counted as cache hits).  w = width of the
     reconstruction/flux type (8 FP64, 4 FP32-mixed); state stays FP64.  The
     reconstruction reads the RBF coefficients and the neighbour-centre offsets
-    (slot_e) only for cells on the full path."""
+    (slot_e) only for cells on the full path.  ngp > 0 (flux quadrature
+    points): the reconstruction also reads each point's coefficients (full
+    path) and offsets and writes each point's states; the flux reads those
+    states and offsets instead of the face-centre states."""
     nc, nf, nb, sS, sC = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"]
     sR, sfull = st["sum_int_coef_full"], st["int_slots_full"]
     ns = 2 * nf  # internal face slots
@@ -74,6 +77,9 @@ def kernel_bytes(st, w=8):
     flux = 8 * nf + 64 * nf + 8 * w * nf + nc * (4 * w + 4 * w + 8 * w + 1) + 32 * nb + 4 * w * nb + 32 * (nf + nb)
     update = nc + 4 * sC + 32 * (nf + nb) + 32 * nc * 2 + 8 * nc + 65 * nb + 32 * nc + 4 * w * (nc + nb) + 32 * nc / 3
     cfl = 8 * nf + 48 * nf + 4 * w * nc
+    if ngp:
+        recon += ngp * (w * sR + 4 * w * ns + 32 * ns)
+        flux += ngp * (8 * w + 32) * nf - 8 * w * nf
     nblk = 8 * (((nf + nb + 255) // 256 + 7) // 8)  # flux grid: one CFL partial per block
     return dict(cfl=cfl, reconstruct=recon, bvd=bvd, flux=flux, update=update, flux_dt=flux + 8 * nblk,
                 dt_reduce=8 * nblk)
@@ -198,6 +204,9 @@ def main():
                          "the face work compacted; 0 automatic")
     ap.add_argument("--cell-order", type=int, default=0, choices=[0, 1],
                     help="device cell order: 0 compact tiles along a Hilbert curve, 1 the mesh's (RCM) order")
+    ap.add_argument("--face-quadrature", type=int, default=0, choices=[0, 1],
+                    help="flux at the face centre (0, the reference) or at the fOrder 2 rule's two points per face "
+                         "(1, extension)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--faithful-boundary", action="store_true",
                     help="riemann case without the boundaryFirstOrder extension (loses positivity)")
@@ -253,7 +262,7 @@ def main():
     # the boundaryFirstOrder extension so the timed steps act on a physical state.
     bfo = args.case == "riemann" and not args.faithful_boundary
     sc = A.schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=args.precision, tile=args.tile, boundaryFirstOrder=bfo,
-                   reconLanes=args.recon_lanes, cellOrder=args.cell_order)
+                   reconLanes=args.recon_lanes, cellOrder=args.cell_order, faceQuadrature=args.face_quadrature)
     mesh.precompute(sc)
     t_pre = time.perf_counter() - t0
     print("bench: %s, nc=%d, precompute %.1f s" % (args.case, mesh.nc, t_pre), file=sys.stderr, flush=True)
@@ -300,7 +309,8 @@ def main():
     t_end = gpu.device_time()
     prof = gpu.profile(reps=20)  # back-to-back launches per kernel; after the timed run
     w = 8 if args.precision == "fp64" else 4
-    kb = kernel_bytes(st, w)
+    ngp = 2 if args.face_quadrature else 0
+    kb = kernel_bytes(st, w, ngp)
     kernels = {}
     for k, (avg, cnt) in prof.items():
         kernels[k] = dict(avg_ms=avg, count_per_step=cnt, alg_bytes=kb[k], GBps=kb[k] / (avg * 1e-3) / 1e9)
@@ -311,7 +321,10 @@ def main():
     balg = survey_balg_per_cell_step(st, w)
     roofline = dict(bound="hbm", achieved=round(d["GBps"], 1), peak=PEAK_HBM_GBPS, unit="GB/s",
                     frac=round(d["GBps"] / PEAK_HBM_GBPS, 4),
-                    traffic=measured_traffic(dom, args.case, args.precision), kernel=dom,
+                    # the committed counter summaries cover the default configuration only
+                    traffic=(measured_traffic(dom, args.case, args.precision)
+                             if not args.face_quadrature and args.cell_order == 0 and args.tile == 64
+                             and args.recon_lanes in (0, 4) else None), kernel=dom,
                     alg_bytes_per_launch=int(d["alg_bytes"]), avg_launch_ms=round(d["avg_ms"], 5))
     cpu = None
     if not args.no_cpu:
@@ -325,7 +338,8 @@ def main():
                    "boundary_faces": st["nb"], "mean_stencil": round(st["meanS"], 3),
                    "mean_compact": round(st["meanC"], 3), "precision": args.precision, "cfl": 0.5,
                    "tile": args.tile, "recon_lanes": args.recon_lanes or "auto",
-                   "cell_order": ["compact tiles (Hilbert)", "mesh (RCM)"][args.cell_order], "device": dev_name.strip(),
+                   "cell_order": ["compact tiles (Hilbert)", "mesh (RCM)"][args.cell_order],
+                   "flux_points_per_face": ngp or 1, "device": dev_name.strip(),
                    "precompute_s": round(t_pre, 1),
                    "boundary_first_order": bfo, "state_positive_after_timed_steps": finite,
                    "t_start": t_start, "untimed_steps_to_t_start": pre_steps, "t_after_timed_steps": round(t_end, 5),
