@@ -164,7 +164,9 @@ __device__ __forceinline__ void limiter_process(R& lim, R& phif, R phic, R phiMa
     lim = fmin(lim, yy);
 }
 
-template <typename R, int TILE>
+// GP: the flux-quadrature extension's point states (a separate instantiation,
+// so the reference path carries none of its registers)
+template <typename R, int TILE, bool GP>
 __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
@@ -236,7 +238,8 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
             o.w = wc.w + gTx * dx + gTy * dy;
             p.fv[2 * f + isNei] = o;
             p.fvT[2 * f + isNei] = o.w;
-            for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
+            if constexpr (GP)
+                for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
                 const double4 dg = p.fgd[(size_t)g * nf + f];
                 const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
                 v4 og;
@@ -294,6 +297,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
         isNotTrouble = phif > (R)0;
         p.fv[2 * f + isNei] = o;
         p.fvT[2 * f + isNei] = o.w;
+        if constexpr (GP)
         for (int g = 0; g < p.ngp; ++g) {  // flux points (extension), as in k_reconstruct_q
             const R* ag = p.rbf_gp + sell(c, (g * maxC + fi) * (maxS + 1), p.ngp * KR);
             R cg = ag[0];
@@ -360,12 +364,14 @@ template <int X, typename T> __device__ __forceinline__ T quad_xor(T x) {
 #ifndef AFAR_RECONQ_WPE_F
 #define AFAR_RECONQ_WPE_F 8
 #endif
-template <typename R> struct ReconQWaves {
-    static constexpr int value = sizeof(R) == 8 ? AFAR_RECONQ_WPE_D : AFAR_RECONQ_WPE_F;
+// The flux-quadrature instantiation holds more values and asks for one wave
+// fewer, so that it does not spill.
+template <typename R, bool GP> struct ReconQWaves {
+    static constexpr int value = (sizeof(R) == 8 ? AFAR_RECONQ_WPE_D : AFAR_RECONQ_WPE_F) - (GP ? 1 : 0);
 };
-#define AFAR_RECONQ_ATTR __attribute__((amdgpu_waves_per_eu(ReconQWaves<R>::value, 8)))
+#define AFAR_RECONQ_ATTR __attribute__((amdgpu_waves_per_eu(ReconQWaves<R, GP>::value, 8)))
 
-template <typename R, int TILE>
+template <typename R, int TILE, bool GP>
 __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(DevPtrs<R> p) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
@@ -492,7 +498,8 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
             o.w = wc.w + gTx * dx + gTy * dy;
             p.fv[2 * f + isNei] = o;
             p.fvT[2 * f + isNei] = o.w;
-            for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
+            if constexpr (GP)
+                for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
                 const double4 dg = p.fgd[(size_t)g * nf + f];
                 const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
                 v4 og;
@@ -555,6 +562,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         p.fvT[2 * f + isNei] = o.w;
         // flux points (extension): RBF sums with the point's coefficients,
         // or the unlimited linear state where the compact range is flat
+        if constexpr (GP)
         for (int g = 0; g < p.ngp; ++g) {
             const R* ag = p.rbf_gp + sell(c, (g * maxC + k) * (maxS + 1), p.ngp * KR);
             R cg = ag[0];
@@ -1028,7 +1036,7 @@ __device__ __forceinline__ void roe_flux(R& rhoFlux, R& rhoUxFlux, R& rhoUyFlux,
 // DT: also reduce the CFL bound of every internal face to one partial minimum
 // per block (dt_part[blockIdx.x]); the bound is a function of the
 // primitive state alone, which no kernel of the stage has changed yet.
-template <typename R, bool DT>
+template <typename R, bool DT, bool GP>
 __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double Cv, double CFL) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
@@ -1103,7 +1111,7 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
             out2 = rhoUyFlux;
             out3 = rhoEFlux;
         };
-        if (p.ngp == 0) {
+        if constexpr (!GP) {
             R rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux;
             point_flux(dxL, dyL, dxR, dyR, p.fv[2 * f], p.fv[2 * f + 1], rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
             p.flux[f] = double4{(double)rhoFlux, (double)rhoUxFlux, (double)rhoUyFlux, (double)rhoEFlux};
@@ -1324,6 +1332,23 @@ void launch_cfl(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream
     hipLaunchKernelGGL(k_cfl<R>, dim3(nt), dim3(256), 0, s, p, Rg, Cv, CFL, chunk);
 }
 
+template <typename R, bool GP>
+void launch_reconstruct_t(const DevPtrs<R>& p, int nb, int tile, size_t lds, hipStream_t s, int lanes) {
+    if (lanes == 4) {
+        switch (tile) {
+            case 64: hipLaunchKernelGGL((k_reconstruct_q<R, 64, GP>), dim3(nb), dim3(256), lds, s, p); break;
+            case 128: hipLaunchKernelGGL((k_reconstruct_q<R, 128, GP>), dim3(nb), dim3(512), lds, s, p); break;
+            default: hipLaunchKernelGGL((k_reconstruct_q<R, 256, GP>), dim3(nb), dim3(1024), lds, s, p); break;
+        }
+        return;
+    }
+    switch (tile) {
+        case 64: hipLaunchKernelGGL((k_reconstruct<R, 64, GP>), dim3(nb), dim3(64), lds, s, p); break;
+        case 128: hipLaunchKernelGGL((k_reconstruct<R, 128, GP>), dim3(nb), dim3(128), lds, s, p); break;
+        default: hipLaunchKernelGGL((k_reconstruct<R, 256, GP>), dim3(nb), dim3(256), lds, s, p); break;
+    }
+}
+
 template <typename R>
 void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s, int lanes) {
     const int nb = pad8(p.ntiles);
@@ -1334,19 +1359,11 @@ void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s
             hipLaunchKernelGGL((k_reconstruct_c<R, 128>), dim3(nb), dim3(512), lds, s, p);
         return;
     }
-    if (lanes == 4) {
-        switch (tile) {
-            case 64: hipLaunchKernelGGL((k_reconstruct_q<R, 64>), dim3(nb), dim3(256), lds, s, p); break;
-            case 128: hipLaunchKernelGGL((k_reconstruct_q<R, 128>), dim3(nb), dim3(512), lds, s, p); break;
-            default: hipLaunchKernelGGL((k_reconstruct_q<R, 256>), dim3(nb), dim3(1024), lds, s, p); break;
-        }
+    if (p.ngp > 0) {
+        launch_reconstruct_t<R, true>(p, nb, tile, lds, s, lanes);
         return;
     }
-    switch (tile) {
-        case 64: hipLaunchKernelGGL((k_reconstruct<R, 64>), dim3(nb), dim3(64), lds, s, p); break;
-        case 128: hipLaunchKernelGGL((k_reconstruct<R, 128>), dim3(nb), dim3(128), lds, s, p); break;
-        default: hipLaunchKernelGGL((k_reconstruct<R, 256>), dim3(nb), dim3(256), lds, s, p); break;
-    }
+    launch_reconstruct_t<R, false>(p, nb, tile, lds, s, lanes);
 }
 
 template <typename R>
@@ -1363,7 +1380,10 @@ int flux_blocks(int nf, int nb) { return pad8(std::max(1, (nf + nb + 255) / 256)
 
 template <typename R>
 void launch_flux(const DevPtrs<R>& p, double Rg, double Cv, hipStream_t s) {
-    hipLaunchKernelGGL((k_flux<R, false>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, 0.0);
+    if (p.ngp > 0)
+        hipLaunchKernelGGL((k_flux<R, false, true>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, 0.0);
+    else
+        hipLaunchKernelGGL((k_flux<R, false, false>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, 0.0);
 }
 
 template <typename R>
@@ -1383,7 +1403,10 @@ void launch_residual_norm(const DevPtrs<R>& p, double* out, hipStream_t s) {
 
 template <typename R>
 void launch_flux_dt(const DevPtrs<R>& p, double Rg, double Cv, double CFL, hipStream_t s) {
-    hipLaunchKernelGGL((k_flux<R, true>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, CFL);
+    if (p.ngp > 0)
+        hipLaunchKernelGGL((k_flux<R, true, true>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, CFL);
+    else
+        hipLaunchKernelGGL((k_flux<R, true, false>), dim3(flux_blocks(p.nf, p.nb)), dim3(256), 0, s, p, Rg, Cv, CFL);
 }
 
 template <typename R>
@@ -1396,10 +1419,13 @@ void launch_update(const DevPtrs<R>& p, double Rg, double gamma, double dt, cons
 template <typename R>
 hipError_t set_lds_limits(size_t lds) {
     hipError_t e = hipSuccess;
-    const void* fns[] = {(const void*)k_reconstruct<R, 64>,   (const void*)k_reconstruct<R, 128>,
-                         (const void*)k_reconstruct<R, 256>,  (const void*)k_reconstruct_q<R, 64>,
-                         (const void*)k_reconstruct_q<R, 128>, (const void*)k_reconstruct_q<R, 256>,
-                         (const void*)k_reconstruct_c<R, 64>, (const void*)k_reconstruct_c<R, 128>};
+    const void* fns[] = {(const void*)k_reconstruct<R, 64, false>,    (const void*)k_reconstruct<R, 128, false>,
+                         (const void*)k_reconstruct<R, 256, false>,   (const void*)k_reconstruct_q<R, 64, false>,
+                         (const void*)k_reconstruct_q<R, 128, false>, (const void*)k_reconstruct_q<R, 256, false>,
+                         (const void*)k_reconstruct<R, 64, true>,     (const void*)k_reconstruct<R, 128, true>,
+                         (const void*)k_reconstruct<R, 256, true>,    (const void*)k_reconstruct_q<R, 64, true>,
+                         (const void*)k_reconstruct_q<R, 128, true>,  (const void*)k_reconstruct_q<R, 256, true>,
+                         (const void*)k_reconstruct_c<R, 64>,         (const void*)k_reconstruct_c<R, 128>};
     for (const void* fn : fns) {
         // the dynamic limit plus the kernel's static LDS may not exceed the block limit
         hipFuncAttributes attr;
