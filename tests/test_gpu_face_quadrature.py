@@ -110,3 +110,27 @@ def test_quadrature_mixed_close_to_fp64(afar):
     say("  rho rel L1: face centre %.3e, two points %.3e" % (rel[0], rel[1]))
     # the points add FP32 sums of their own; the error stays of the face-centre path's size
     assert rel[1] < 1e-4 and rel[1] < 3.0 * rel[0]
+
+
+def test_quadrature_batch_equals_alone(afar):
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    from afar_cudaeuler2dmixo3rk3solver_amd.batch import BatchSolver
+    from afar_cudaeuler2dmixo3rk3solver_amd.solver import BasicCUDASolver
+    say("case: batch of 3 riemann meshes with two flux points (FP64), 20 steps, vs each alone")
+    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, boundaryFirstOrder=True, faceQuadrature=1)
+    meshes, W0s = [], []
+    for n, seed in ((24, 1), (40, 2), (32, 3)):
+        m, w = cases.riemann_case(n=n, seed=seed)
+        m.precompute(sc)
+        meshes.append(m)
+        W0s.append(w)
+    b = BatchSolver(meshes, sc, W0s)
+    b.evolve(20)
+    b.synchronize()
+    dts = b.last_dt()
+    for k, (m, w) in enumerate(zip(meshes, W0s)):
+        g = BasicCUDASolver(m, sc, w)
+        g.evolve_auto(20)
+        assert g.last_dt() == dts[k]
+        assert np.array_equal(g.conserved(), b.conserved(k)), k
+        g.close()
