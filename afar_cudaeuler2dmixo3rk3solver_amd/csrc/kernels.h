@@ -37,6 +37,7 @@ struct DevPtrs {
     typename VT<R>::v4* lim;   // nc limiters p, Ux, Uy, T
     int8_t* smooth;            // nc+nb smoothness flag (shockIndicator[2c])
     int8_t* ftype;             // nc final reconstruction type (shockIndicator[2c+1])
+    uint8_t* flat_prev;        // nc: the cell took the flat branch at the last reconstruction (load scheduling only)
     const uint8_t* scnt;       // nc stencil size
     const uint8_t* ccnt;       // nc compact stencil size
     const uint16_t* sten;      // sliced ELL, LDS-local stencil ids

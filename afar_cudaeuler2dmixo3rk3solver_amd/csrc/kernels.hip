@@ -389,10 +389,11 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     // halo ids and stencil block; (2) lane k's least-squares weight and slot
     // data, and the halo states.  Their latencies overlap.
     const bool inb = c < p.nc;
-    int S = 0, C = 0, sf = 0;
+    int S = 0, C = 0, sf = 0, wasFlat = 0;
     if (inb) {
         S = p.scnt[c];
         C = p.ccnt[c];
+        wasFlat = p.flat_prev[c];
     }
     const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
     const int wpt = TILE >> 6;  // stencil block: (waves of the tile) * maxS * 64 ids
@@ -415,6 +416,19 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     }
     v4 wHalo;
     if (t < hn) wHalo = p.W[h0];
+    // A cell that took the full path at the previous launch most likely takes
+    // it again: its RBF coefficients and centre offset are requested in this
+    // round already (flat_prev only schedules loads; results do not depend on it).
+    R cf[kMaxSten + 1];
+    double2 e = {0.0, 0.0};
+    const R* a = p.rbf + sell(c, k * (maxS + 1), KR);  // row stride 64
+    const bool early = inb && k < C && !wasFlat;
+    if (early) {
+        e = p.slot_e[sell(c, k, maxC)];
+#pragma unroll
+        for (int i = 0; i <= kMaxSten; ++i)
+            if (i <= S) cf[i] = a[(size_t)i * 64];
+    }
     uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
     v2* sM = reinterpret_cast<v2*>(smem_raw + p.lds_m_off);  // 4 weights per cell
     uint4* stDst = reinterpret_cast<uint4*>(sSt);
@@ -462,12 +476,10 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     const bool live = k < C && f < nf;
     const bool flat = fabs(TMax - TMin) < K<R>::VSMALL;
 
-    // Full path: every RBF coefficient of the face and its centre offset are
-    // requested at once (one memory round trip).
-    R cf[kMaxSten + 1];
-    double2 e = {0.0, 0.0};
-    const R* a = p.rbf + sell(c, k * (maxS + 1), KR);  // row stride 64
-    if (!flat && live) {
+    if (k == 0) p.flat_prev[c] = flat;
+    // Full path not foreseen: every RBF coefficient of the face and its centre
+    // offset are requested now, at once (one memory round trip).
+    if (!flat && live && !early) {
         e = p.slot_e[sell(c, k, maxC)];
 #pragma unroll
         for (int i = 0; i <= kMaxSten; ++i)

@@ -457,6 +457,9 @@ class SolverImpl final : public DeviceSolver {
         m_ftype.alloc(nc);
         P_.ftype = m_ftype.as<int8_t>();
         check_hip(hipMemset(P_.ftype, 0, nc), "hipMemset");
+        m_flat_prev.alloc(nc);
+        P_.flat_prev = m_flat_prev.as<uint8_t>();
+        check_hip(hipMemset(P_.flat_prev, 0, nc), "hipMemset");
         P_.scnt = put(m_scnt, L_.scnt.data(), nc);
         P_.ccnt = put(m_ccnt, L_.ccnt.data(), nc);
         P_.sten = put(m_sten, L_.sten.data(), L_.sten.size());
@@ -554,7 +557,7 @@ class SolverImpl final : public DeviceSolver {
     int lanes_ = 1;  // threads per cell of the reconstruction kernel
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fv, m_flux, m_bcell,
-        m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res, m_rbf_gp, m_fgd, m_fvg;
+        m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res, m_rbf_gp, m_fgd, m_fvg, m_flat_prev;
 };
 
 // A solver on relabelled cells and faces (compact_tile_order) that reads and
