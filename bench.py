@@ -110,8 +110,8 @@ def measured_traffic(kernel, case, precision):
     return None
 
 
-KERNEL_KEYS = [("k_reconstruct", "reconstruct"), ("k_bvd", "bvd"), ("k_flux<double, true>", "flux_dt"),
-               ("k_flux<float, true>", "flux_dt"), ("k_flux<", "flux"), ("k_dt_reduce", "dt_reduce"),
+KERNEL_KEYS = [("k_reconstruct", "reconstruct"), ("k_bvd", "bvd"), ("k_flux<double, true", "flux_dt"),
+               ("k_flux<float, true", "flux_dt"), ("k_flux<", "flux"), ("k_dt_reduce", "dt_reduce"),
                ("k_update", "update"), ("k_cfl", "cfl")]
 
 
