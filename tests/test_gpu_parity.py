@@ -179,6 +179,33 @@ def test_evolve_auto_equals_host_dt(afar):
     assert np.array_equal(a.conserved(), b.conserved())
 
 
+def test_graph_replays_each_equal_eager(afar):
+    """Every replay of the captured step graph equals the eager step (host
+    time step) bit for bit, with eager work of another solver and host
+    synchronisations between the replays."""
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    from afar_cudaeuler2dmixo3rk3solver_amd.solver import BasicCUDASolver
+    say("case: corner 80x40, 8 single-step graph replays vs eager steps, eager work in between")
+    mesh, W0 = cases.corner_case(nx=80, ny=40)
+    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5)
+    mesh.precompute(sc)
+    a = BasicCUDASolver(mesh, sc, W0)  # eager
+    b = BasicCUDASolver(mesh, sc, W0)  # graph
+    c = BasicCUDASolver(mesh, sc, W0)  # unrelated eager work
+    for i in range(8):
+        dt_a = a.set_delta_t()
+        a.evolve(dt_a)
+        a.synchronize()
+        b.evolve_auto(1)
+        c.evolve(c.set_delta_t())
+        b.synchronize()
+        assert b.last_dt() == dt_a, i
+        assert np.array_equal(a.conserved(), b.conserved()), i
+        c.synchronize()
+    for g in (a, b, c):
+        g.close()
+
+
 def test_riemann_boundary_first_order_to_t03(afar):
     """Mixed mesh, four-quadrant problem with the boundaryFirstOrder extension,
     run to t = 0.3 on the device and on the oracle: bitwise equal, positive."""
