@@ -71,3 +71,22 @@ def test_polygon_mesh_equals_generator(afar):
     m = afar.Mesh.from_polygons(pts, np.arange(0, 4 * n * n + 1, 4), cells, be, bp, [4, 4, 4, 4])
     assert np.array_equal(m.permutation(), g.permutation())
     assert np.array_equal(m.faces()["face"], g.faces()["face"])
+
+
+def test_stencil_over_twenty_is_an_error(afar):
+    """A vertex shared by 26 triangles gives their edgeNeighbour stencils more
+    than 20 cells: the precompute reports it (the reference exits,
+    basicCUDAsolver.cpp:393-397) and the library stays usable."""
+    k = 26
+    ang = np.arange(k) * 2 * np.pi / k
+    pts = np.vstack([[0.0, 0.0], np.stack([np.cos(ang), np.sin(ang)], 1)])
+    cells = []
+    for i in range(k):
+        cells += [0, 1 + i, 1 + (i + 1) % k]
+    be = [[1 + i, 1 + (i + 1) % k] for i in range(k)]
+    m = afar.Mesh.from_polygons(pts, np.arange(0, 3 * k + 1, 3), cells, be, [0] * k, [4])
+    with pytest.raises(afar.AfarError, match="Too large stencil"):
+        m.precompute(afar.schemes(cOrder=7, fOrder=2))
+    # a valid mesh still builds afterwards
+    ok = afar.Mesh.riemann(n=6, seed=2)
+    ok.precompute(afar.schemes(cOrder=7, fOrder=2))
