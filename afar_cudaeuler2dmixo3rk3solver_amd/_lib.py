@@ -125,6 +125,7 @@ SIGNATURES = {
     "afar_solver_set_conserved": ([_VP, _D], C.c_int),
     "afar_solver_time_steps": ([_VP, C.c_int32, C.c_int32, _D], C.c_int),
     "afar_solver_profile": ([_VP, C.c_int32, _D, _I32], C.c_int),
+    "afar_solver_time_steps_kernels": ([_VP, C.c_int32, _D, _D, _I32], C.c_int),
     "afar_solver_sizes": ([_VP, _I64], C.c_int),
     "afar_solver_run_stage": ([_VP, C.c_int32, C.c_double, C.c_int32], C.c_int),
     "afar_solver_fetch": ([_VP, C.c_char_p, _VP, C.c_size_t], C.c_int),

@@ -831,6 +831,16 @@ int afar_solver_time_steps(afar_solver* h, int32_t nsteps, int32_t use_graph, do
     });
 }
 
+int afar_solver_time_steps_kernels(afar_solver* h, int32_t nsteps, double* ms_total, double* ms_by_kind,
+                                   int32_t* launches_by_kind) {
+    return guard([&] {
+        need(h && ms_total && ms_by_kind && launches_by_kind && nsteps >= 0, "bad arguments");
+        int c[kKernKinds];
+        *ms_total = h->dev->time_steps_kernels(nsteps, ms_by_kind, c);
+        for (int k = 0; k < kKernKinds; ++k) launches_by_kind[k] = c[k];
+    });
+}
+
 int afar_solver_profile(afar_solver* h, int32_t reps, double* ms, int32_t* count) {
     return guard([&] {
         need(h && ms && count && reps > 0, "bad arguments");

@@ -276,6 +276,69 @@ class SolverImpl final : public DeviceSolver {
         return ms;
     }
 
+    // nsteps device-time-step RK3 steps as graph replays, with an event before
+    // and after every kernel inside the graph: the total time (returned, ms)
+    // and each kind's summed kernel time and launch count over these very
+    // steps.  Two graph copies with their own events alternate, so the events
+    // of one step are read while the next runs; the device never waits for
+    // the host.
+    double time_steps_kernels(int nsteps, double* ms_kind, int* cnt_kind) override {
+        constexpr int NK = 13;
+        static const int kind[NK] = {kKernRecon, kKernBvd, kKernFluxDt, kKernDtReduce, kKernUpdate,
+                                     kKernRecon, kKernBvd, kKernFlux,   kKernUpdate,   kKernRecon,
+                                     kKernBvd,   kKernFlux, kKernUpdate};
+        for (int k = 0; k < kKernKinds; ++k) {
+            ms_kind[k] = 0.0;
+            cnt_kind[k] = 0;
+        }
+        if (nsteps <= 0) return 0.0;
+        struct Copy {
+            hipEvent_t ev[NK + 1] = {};
+            hipGraph_t g = nullptr;
+            hipGraphExec_t ge = nullptr;
+        } cp[2];
+        struct Cleanup {
+            Copy* c;
+            ~Cleanup() {
+                for (int b = 0; b < 2; ++b) {
+                    if (c[b].ge) (void)hipGraphExecDestroy(c[b].ge);
+                    if (c[b].g) (void)hipGraphDestroy(c[b].g);
+                    for (hipEvent_t e : c[b].ev)
+                        if (e) (void)hipEventDestroy(e);
+                }
+            }
+        } cleanup{cp};
+        synchronize();
+        for (int b = 0; b < 2; ++b) {
+            for (hipEvent_t& e : cp[b].ev) check_hip(hipEventCreate(&e), "hipEventCreate");
+            check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            check_hip(hipEventRecordWithFlags(cp[b].ev[0], stream_, hipEventRecordExternal), "hipEventRecord");
+            enqueue_stages(0.0, P_.dt, cp[b].ev + 1);
+            check_hip(hipStreamEndCapture(stream_, &cp[b].g), "hipStreamEndCapture");
+            check_hip(hipGraphInstantiate(&cp[b].ge, cp[b].g, nullptr, nullptr, 0), "hipGraphInstantiate");
+        }
+        auto collect = [&](Copy& c) {
+            check_hip(hipEventSynchronize(c.ev[NK]), "hipEventSynchronize");
+            for (int j = 0; j < NK; ++j) {
+                float ms = 0.f;
+                check_hip(hipEventElapsedTime(&ms, c.ev[j], c.ev[j + 1]), "hipEventElapsedTime");
+                ms_kind[kind[j]] += ms;
+                cnt_kind[kind[j]] += 1;
+            }
+        };
+        check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
+        for (int i = 0; i < nsteps; ++i) {
+            if (i >= 2) collect(cp[i & 1]);  // step i-2 used this copy
+            check_hip(hipGraphLaunch(cp[i & 1].ge, stream_), "hipGraphLaunch");
+        }
+        check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
+        for (int i = std::max(0, nsteps - 2); i < nsteps; ++i) collect(cp[i & 1]);
+        check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
+        float total = 0.f;
+        check_hip(hipEventElapsedTime(&total, ev0_, ev1_), "hipEventElapsedTime");
+        return total;
+    }
+
     // Average duration of each kernel: `reps` back-to-back launches of that
     // kernel between two events on the solver's stream (the bvd and update
     // launches keep transforming the state, so call this after timed work).
@@ -381,17 +444,29 @@ class SolverImpl final : public DeviceSolver {
     // time step is evaluated on the device by the first flux pass (from the
     // state the step starts from, exactly as the standalone CFL pass would)
     // and read by the updates from dt_ptr.
-    void enqueue_stages(double dt, const double* dt_ptr) {
+    // One RK3 step.  ev (optional): an event recorded after every launch, in
+    // launch order (13 with a device time step).
+    void enqueue_stages(double dt, const double* dt_ptr, hipEvent_t* ev = nullptr) {
+        int j = 0;
+        auto mark = [&] {
+            if (ev) check_hip(hipEventRecordWithFlags(ev[j++], stream_, hipEventRecordExternal), "hipEventRecord");
+        };
         for (int s = 0; s < 3; ++s) {
             launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_, lanes_);
+            mark();
             launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
+            mark();
             if (s == 0 && dt_ptr) {
                 launch_flux_dt<R>(P_, cfg_.R, cfg_.Cv, cfg_.CFL, stream_);
+                mark();
                 launch_dt_reduce<R>(P_, stream_);
+                mark();
             } else {
                 launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
+                mark();
             }
             launch_update<R>(P_, cfg_.R, gamma(), dt, dt_ptr, b1(s), b2(s), b3(s), s == 2, stream_);
+            mark();
         }
     }
 
@@ -628,6 +703,9 @@ class RelabeledSolver final : public DeviceSolver {
     void get_time(double* t, double* steps) override { in_->get_time(t, steps); }
     double residual_norm() override { return in_->residual_norm(); }
     double time_steps(int nsteps, bool graph) override { return in_->time_steps(nsteps, graph); }
+    double time_steps_kernels(int nsteps, double* ms_kind, int* cnt_kind) override {
+        return in_->time_steps_kernels(nsteps, ms_kind, cnt_kind);
+    }
     void profile(double* ms, int* count, int reps) override { in_->profile(ms, count, reps); }
     const DevLayout& layout() const override { return in_->layout(); }
     // Cell arrays (W, U, gradients, lim, smooth, ftype) and face arrays (fv,

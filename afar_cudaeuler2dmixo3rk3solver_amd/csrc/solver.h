@@ -68,6 +68,9 @@ class DeviceSolver {
     // switch flags become those of the current state).
     virtual double residual_norm() = 0;
     virtual double time_steps(int nsteps, bool graph) = 0;  // hipEvent milliseconds for nsteps
+    // nsteps graph-replayed steps with events around every kernel: total ms,
+    // and per kernel kind the summed ms and launch count over those steps
+    virtual double time_steps_kernels(int nsteps, double* ms_kind, int* cnt_kind) = 0;
     virtual void profile(double* ms, int* count, int reps) = 0;
     virtual const DevLayout& layout() const = 0;
     virtual void debug_fetch(const char* what, void* out, size_t bytes) = 0;

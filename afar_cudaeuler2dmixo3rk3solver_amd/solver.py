@@ -389,6 +389,19 @@ class BasicCUDASolver:
         check(lib().afar_solver_time_steps(self._h, nsteps, int(graph), C.byref(ms)))
         return ms.value
 
+    def time_steps_kernels(self, nsteps):
+        """nsteps graph-replayed steps with events around every kernel:
+        (total ms, {kernel: (average ms per launch, launches per step)})."""
+        names = ["cfl", "reconstruct", "bvd", "flux", "update", "flux_dt", "dt_reduce"]
+        tot = C.c_double()
+        ms = np.zeros(len(names))
+        cnt = np.zeros(len(names), dtype=np.int32)
+        check(lib().afar_solver_time_steps_kernels(self._h, nsteps, C.byref(tot), dptr(ms),
+                                                   cnt.ctypes.data_as(C.POINTER(C.c_int32))))
+        self.steps += nsteps
+        return tot.value, {n: (ms[i] / cnt[i] if cnt[i] else 0.0, int(cnt[i]) // max(nsteps, 1))
+                           for i, n in enumerate(names)}
+
     def profile(self, reps=1):
         names = ["cfl", "reconstruct", "bvd", "flux", "update", "flux_dt", "dt_reduce"]
         ms = np.zeros(len(names))

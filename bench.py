@@ -307,13 +307,21 @@ def main():
     finite = bool(np.isfinite(W_end).all() and (W_end[:, 0] > 0).all() and (W_end[:, 3] > 0).all())
     st = mesh_stats(mesh, field, args.precision)
     t_end = gpu.device_time()
-    prof = gpu.profile(reps=20)  # back-to-back launches per kernel; after the timed run
+    # Per-kernel durations: the same number of further steps, graph-replayed
+    # with an event after every kernel (each kernel's time in the running
+    # pipeline, inter-kernel gap included); then back-to-back launches of each
+    # kernel alone, reported beside it (and the only figure for the
+    # standalone CFL pass, which a device-time-step RK3 step does not run).
+    live_ms, live = gpu.time_steps_kernels(args.steps)
+    b2b = gpu.profile(reps=20)
     w = 8 if args.precision == "fp64" else 4
     ngp = 2 if args.face_quadrature else 0
     kb = kernel_bytes(st, w, ngp)
     kernels = {}
-    for k, (avg, cnt) in prof.items():
-        kernels[k] = dict(avg_ms=avg, count_per_step=cnt, alg_bytes=kb[k], GBps=kb[k] / (avg * 1e-3) / 1e9)
+    for k, (avg_b2b, cnt) in b2b.items():
+        avg = live[k][0] if live[k][1] else avg_b2b
+        kernels[k] = dict(avg_ms=avg, count_per_step=cnt, alg_bytes=kb[k], GBps=kb[k] / (avg * 1e-3) / 1e9,
+                          back_to_back_ms=avg_b2b)
     # cfl: the standalone pass of host-requested time steps; a device-time-step
     # RK3 step gets its time step from flux_dt + dt_reduce instead (count 0)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count_per_step"])
@@ -325,7 +333,9 @@ def main():
                     traffic=(measured_traffic(dom, args.case, args.precision)
                              if not args.face_quadrature and args.cell_order == 0 and args.tile == 64
                              and args.recon_lanes in (0, 4) else None), kernel=dom,
-                    alg_bytes_per_launch=int(d["alg_bytes"]), avg_launch_ms=round(d["avg_ms"], 5))
+                    alg_bytes_per_launch=int(d["alg_bytes"]), avg_launch_ms=round(d["avg_ms"], 5),
+                    timing="hipEvents after every kernel of %d graph-replayed steps following the timed ones "
+                           "(%.4f ms per step with the events)" % (args.steps, live_ms / max(args.steps, 1)))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, sc, W_start)

@@ -219,6 +219,14 @@ int afar_solver_get_conserved(afar_solver* h, double* U);
 int afar_solver_set_conserved(afar_solver* h, const double* U);
 /* hipEvent time of nsteps auto-dt steps on the solver's stream (graph replay if use_graph). */
 int afar_solver_time_steps(afar_solver* h, int32_t nsteps, int32_t use_graph, double* ms);
+/* nsteps auto-dt steps replayed as a hipGraph with an event after every
+ * kernel (two graph copies alternate, so the host reads one step's events
+ * while the next runs): *ms_total for the nsteps, and per kernel kind (the
+ * order of afar_solver_profile) the summed kernel time ms_by_kind[7] and the
+ * number of launches launches_by_kind[7] over those same steps.  Extension
+ * without a reference counterpart: per-kernel timing of the timed steps. */
+int afar_solver_time_steps_kernels(afar_solver* h, int32_t nsteps, double* ms_total, double* ms_by_kind,
+                                   int32_t* launches_by_kind);
 /* Average time of `reps` back-to-back launches of each kernel: ms[7] and
  * launches per device-time-step RK3 step count[7] for {cfl (standalone, used
  * by set_delta_t), reconstruct, bvd, flux, update, flux with CFL partials,
