@@ -566,7 +566,7 @@ int afar_solver_create(const afar_mesh* h, const afar_schemes* s, const double* 
         need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 2 || s->reconLanes == 4,
              "reconLanes must be 0, 1, 2 or 4");
         cfg.reconLanes = s->reconLanes;
-        need(s->cellOrder == 0 || s->cellOrder == 1, "cellOrder must be 0 or 1");
+        need(s->cellOrder >= 0 && s->cellOrder <= 2, "cellOrder must be 0, 1 or 2");
         cfg.cellOrder = s->cellOrder;
         auto sol = std::make_unique<afar_solver>();
         sol->dev = make_device_solver(ra, cfg);
@@ -607,7 +607,7 @@ int afar_batch_create(int32_t n, const afar_mesh* const* meshes, const afar_sche
         need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 2 || s->reconLanes == 4,
              "reconLanes must be 0, 1, 2 or 4");
         cfg.reconLanes = s->reconLanes;
-        need(s->cellOrder == 0 || s->cellOrder == 1, "cellOrder must be 0 or 1");
+        need(s->cellOrder >= 0 && s->cellOrder <= 2, "cellOrder must be 0, 1 or 2");
         cfg.cellOrder = s->cellOrder;
         b->dev = make_device_solver(u, cfg, &b->idx);
         *out = b.release();

@@ -153,7 +153,7 @@ uint64_t hilbert_index(uint32_t n, uint32_t x, uint32_t y) {
 
 }  // namespace
 
-Relabel compact_tile_order(const RefArrays& ra, const BatchIndex& bi) {
+Relabel compact_tile_order(const RefArrays& ra, const BatchIndex& bi, int group) {
     const int nc = ra.nc, nf = ra.nf;
     Relabel r;
     r.cell_old.resize(nc);
@@ -181,6 +181,15 @@ Relabel compact_tile_order(const RefArrays& ra, const BatchIndex& bi) {
         }
         std::sort(r.cell_old.begin() + c0, r.cell_old.begin() + c1,
                   [&](int a, int b) { return key[a] != key[b] ? key[a] < key[b] : a < b; });
+        if (group > 0)
+            for (int b0 = c0; b0 < c1;) {
+                const int b1 = std::min(c1, (b0 / group + 1) * group);
+                std::stable_sort(r.cell_old.begin() + b0, r.cell_old.begin() + b1, [&](int a, int b) {
+                    const int ca = ra.compactStencilSize[a], cb = ra.compactStencilSize[b];
+                    return ca != cb ? ca < cb : ra.stencilSize[a] < ra.stencilSize[b];
+                });
+                b0 = b1;
+            }
     }
     for (int j = 0; j < nc; ++j) r.cell_new[r.cell_old[j]] = j;
     r.face_old.resize(nf);

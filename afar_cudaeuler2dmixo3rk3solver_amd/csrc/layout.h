@@ -107,7 +107,13 @@ struct Relabel {
     std::vector<int32_t> cell_new, cell_old;  // nc: new index of mesh cell i; mesh cell at new index j
     std::vector<int32_t> face_new, face_old;  // nf internal faces, likewise
 };
-Relabel compact_tile_order(const RefArrays& ra, const BatchIndex& bi);
+// group > 0: within each aligned block of `group` cells (inside one
+// instance) cells are further ordered by face count, then stencil size, so
+// that the cells of a wavefront run loops of equal length and read
+// coefficient rows that all of them use.  The price is that cells close in
+// space are no longer close in memory within a block, which the face kernel's
+// gathers of both sides' states pay for.
+Relabel compact_tile_order(const RefArrays& ra, const BatchIndex& bi, int group = 0);
 // The same problem with cells and internal faces renamed by r (pseudo-cells
 // and boundary faces keep their indices).
 RefArrays relabel(const RefArrays& ra, const Relabel& r);

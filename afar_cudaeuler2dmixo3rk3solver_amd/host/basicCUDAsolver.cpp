@@ -197,7 +197,9 @@ afar_schemes readSchemes(const dictionary& fv, const dictionary& control) {
     s.tile = (int32_t)fv.lookupOrDefault("CUDA.tile", 64L);
     s.boundaryFirstOrder = fv.lookupOrDefault("CUDA.boundaryFirstOrder", false);
     s.faceQuadrature = fv.lookupOrDefault("CUDA.faceQuadrature", false);  // extension: flux at the fOrder points
-    s.cellOrder = fv.found("CUDA.cellOrder") && fv.lookupWord("CUDA.cellOrder") == "mesh" ? 1 : 0;
+    // device cell order: grouped compact tiles (default), "mesh", or "curve" (compact tiles, ungrouped)
+    const std::string order = fv.found("CUDA.cellOrder") ? fv.lookupWord("CUDA.cellOrder") : "tiles";
+    s.cellOrder = order == "mesh" ? 1 : order == "curve" ? 2 : 0;
     return s;
 }
 

@@ -166,7 +166,35 @@ def hbm_table(reads_csv, writes_csv, copy_reads_csv, copy_writes_csv, copy_mib=1
         print("%s %d %.0f %.0f %d" % (key, len(acc[key][0]), r, w, int(round(r + w))))
 
 
-def cpu_baseline(mesh, sc, W0, budget_s=12.0, max_steps=40):
+def bandwidth_probe(mib=1024, reps=20):
+    """Practical HBM rates on this GPU: torch's own copy (read + write),
+    read-only sum and fill of `mib` MiB of float64, timed with events on the
+    current stream after three untimed runs each."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.rand(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    nb = n * 8
+    ms = dict(copy=timed(lambda: y.copy_(x)), sum=timed(lambda: x.sum()), fill=timed(lambda: y.fill_(1.0)))
+    print(json.dumps({"bytes": nb, "reps": reps, "device": torch.cuda.get_device_name(0),
+                      "copy_GBps": round(2 * nb / ms["copy"] / 1e6, 1), "sum_read_GBps": round(nb / ms["sum"] / 1e6, 1),
+                      "fill_write_GBps": round(nb / ms["fill"] / 1e6, 1), "ms": {k: round(v, 4) for k, v in ms.items()}}))
+
+
+def cpu_baseline(mesh, sc, W0, budget_s=15.0, max_steps=100):
     """The FP64 oracle (OpenMP) on the same mesh: steps until budget_s."""
     from oracle import pyoracle as O
     ref = mesh.reference_arrays(sc, W0)
@@ -202,8 +230,10 @@ def main():
     ap.add_argument("--recon-lanes", type=int, default=0, choices=[0, 1, 2, 4],
                     help="reconstruction kernel: 1 one thread per cell, 4 four per cell, 2 four per cell with "
                          "the face work compacted; 0 automatic")
-    ap.add_argument("--cell-order", type=int, default=0, choices=[0, 1],
-                    help="device cell order: 0 compact tiles along a Hilbert curve, 1 the mesh's (RCM) order")
+    ap.add_argument("--cell-order", type=int, default=0, choices=[0, 1, 2],
+                    help="device cell order: 0 compact tiles along a Hilbert curve with 32 (FP64) or 64 (mixed) "
+                         "cell blocks ordered by face count and stencil size, 1 the mesh's (RCM) order, "
+                         "2 compact tiles in curve order throughout")
     ap.add_argument("--face-quadrature", type=int, default=0, choices=[0, 1],
                     help="flux at the face centre (0, the reference) or at the fOrder 2 rule's two points per face "
                          "(1, extension)")
@@ -225,6 +255,8 @@ def main():
                     metavar=("READS_CSV", "WRITES_CSV", "COPY_READS_CSV", "COPY_WRITES_CSV"),
                     help="only print HBM bytes per launch per kernel from rocprofv3 --pmc counter_collection.csv "
                          "files (no GPU needed)")
+    ap.add_argument("--bandwidth-probe", action="store_true",
+                    help="only time torch's 1 GiB copy, read-only sum and fill on the GPU (practical HBM rates)")
     args = ap.parse_args()
     if args.hbm_table:
         return hbm_table(*args.hbm_table, copy_mib=args.copy_calibration or 1024)
@@ -232,6 +264,8 @@ def main():
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.bandwidth_probe:
+        return bandwidth_probe()
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 8
@@ -348,7 +382,9 @@ def main():
                    "boundary_faces": st["nb"], "mean_stencil": round(st["meanS"], 3),
                    "mean_compact": round(st["meanC"], 3), "precision": args.precision, "cfl": 0.5,
                    "tile": args.tile, "recon_lanes": args.recon_lanes or "auto",
-                   "cell_order": ["compact tiles (Hilbert)", "mesh (RCM)"][args.cell_order],
+                   "cell_order": ["compact tiles (Hilbert), %d-cell blocks by face count and stencil size"
+                                  % (32 if args.precision == "fp64" else 64), "mesh (RCM)",
+                                  "compact tiles (Hilbert)"][args.cell_order],
                    "flux_points_per_face": ngp or 1, "device": dev_name.strip(),
                    "precompute_s": round(t_pre, 1),
                    "boundary_first_order": bfo, "state_positive_after_timed_steps": finite,

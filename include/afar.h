@@ -66,8 +66,10 @@ typedef struct afar_schemes {
                             tile's face work compacted (2 and 4: cells of at most four faces;
                             2: tiles of at most 128 cells; automatic picks 4, else 1) */
     int32_t cellOrder;   /* device order of cells, same results: 0 compact tiles (cells along a
-                            Hilbert curve through their centres, faces by owner), 1 the mesh's
-                            order; every array the library returns is in the mesh's order */
+                            Hilbert curve through their centres, faces by owner) whose blocks of
+                            32 (FP64) or 64 (mixed) cells are ordered by face count and stencil
+                            size, 1 the mesh's order, 2 compact tiles in curve order throughout;
+                            every array the library returns is in the mesh's order */
     int32_t faceQuadrature; /* extension, 0 = reference: the flux of an internal face is evaluated
                             at its centre (cudaRiemannSolver.h:178-182).  1: at the points of the
                             fOrder bar rule (d2GaussQuadrature.C:153-214), weights summing to

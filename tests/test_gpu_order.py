@@ -1,7 +1,8 @@
 """GPU: the device cell order is a relabelling only.
 
-Cells along a Hilbert curve (compact tiles, the default) and the mesh's own
-order give the same results bit for bit: state, primitive field and flags,
+Cells along a Hilbert curve with each block of 32 (FP64) or 64 (mixed) cells
+ordered by face count and stencil size (the default), the curve order alone,
+and the mesh's own order give the same results bit for bit: state, primitive field and flags,
 time steps, every stage array read back in the mesh's order; the residual
 norm sums in the device's cell order and agrees to rounding.
 """
@@ -15,14 +16,15 @@ def say(msg):
     print(msg, flush=True)
 
 
+@pytest.mark.parametrize("order", [0, 2])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_cell_order_is_relabelling(afar, precision):
+def test_cell_order_is_relabelling(afar, precision, order):
     from afar_cudaeuler2dmixo3rk3solver_amd import cases
     from afar_cudaeuler2dmixo3rk3solver_amd.solver import BasicCUDASolver
-    say("case: riemann 96x96 (%s), compact-tile vs mesh cell order, 30 steps and stage arrays" % precision)
+    say("case: riemann 96x96 (%s), cell order %d vs mesh cell order, 30 steps and stage arrays" % (precision, order))
     mesh, W0 = cases.riemann_case(n=96, seed=1)
     kw = dict(cOrder=7, fOrder=2, maxCo=0.5, precision=precision, boundaryFirstOrder=True)
-    sa, sb = afar.schemes(cellOrder=0, **kw), afar.schemes(cellOrder=1, **kw)
+    sa, sb = afar.schemes(cellOrder=order, **kw), afar.schemes(cellOrder=1, **kw)
     mesh.precompute(sa)
     a = BasicCUDASolver(mesh, sa, W0)
     b = BasicCUDASolver(mesh, sb, W0)
@@ -49,10 +51,11 @@ def test_cell_order_is_relabelling(afar, precision):
     b.close()
 
 
-def test_cell_order_batch(afar):
+@pytest.mark.parametrize("order", [0, 2])
+def test_cell_order_batch(afar, order):
     from afar_cudaeuler2dmixo3rk3solver_amd import cases
     from afar_cudaeuler2dmixo3rk3solver_amd.batch import BatchSolver
-    say("case: batch of 3 corner meshes (mixed), compact-tile vs mesh cell order, 40 steps")
+    say("case: batch of 3 corner meshes (mixed), cell order %d vs mesh cell order, 40 steps" % order)
     specs = [dict(nx=40, ny=20, mach_u=2.0, angle_deg=15.0, seed=0),
              dict(nx=24, ny=12, mach_u=3.5, angle_deg=8.0, seed=5),
              dict(nx=50, ny=30, mach_u=1.6, angle_deg=22.0, seed=9)]
@@ -63,7 +66,7 @@ def test_cell_order_batch(afar):
         m.precompute(afar.schemes(**kw))
         meshes.append(m)
         W0s.append(w)
-    a = BatchSolver(meshes, afar.schemes(cellOrder=0, **kw), W0s)
+    a = BatchSolver(meshes, afar.schemes(cellOrder=order, **kw), W0s)
     b = BatchSolver(meshes, afar.schemes(cellOrder=1, **kw), W0s)
     a.evolve(40)
     b.evolve(40)

@@ -135,6 +135,34 @@ def test_riemann_small_parity(afar):
     assert (orc.final_types < 0).sum() > 0  # the KFVS branch ran
 
 
+def fan_case(afar, k=20):
+    """k triangles around one vertex inside a slip wall: every cell's
+    edgeNeighbour stencil holds the other k - 1 triangles and its ghost, so
+    k = 20 gives the largest stencil the reference accepts (20,
+    basicCUDAsolver.cpp:393-397) in every cell, all in one partial tile."""
+    ang = np.arange(k) * 2 * np.pi / k
+    pts = np.vstack([[0.0, 0.0], np.stack([np.cos(ang), np.sin(ang)], 1)])
+    cells = []
+    for i in range(k):
+        cells += [0, 1 + i, 1 + (i + 1) % k]
+    be = [[1 + i, 1 + (i + 1) % k] for i in range(k)]
+    mesh = afar.Mesh.from_polygons(pts, np.arange(0, 3 * k + 1, 3), cells, be, [0] * k, [2])
+    th = (np.arange(k) + 0.5) * 2 * np.pi / k
+    W0 = np.stack([1.0 + 0.3 * np.cos(th) + 0.1 * np.sin(3 * th), 0.1 * np.sin(th), -0.2 * np.cos(th),
+                   1.0 + 0.2 * np.sin(2 * th)], 1)
+    return mesh, W0
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_max_stencil_fan_parity(afar, lanes):
+    mesh, W0 = fan_case(afar)
+    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, reconLanes=lanes)
+    gpu, orc = run_parity(afar, mesh, W0, sc, 200, "fan of 20 triangles (stencils of 20), lanes %d" % lanes)
+    assert mesh.sizes()["maxS"] == 20
+    assert np.array_equal(gpu.conserved()[mesh.permutation()], orc.U)
+    assert (orc.final_types != 0).any() and (orc.final_types == 0).any()  # both branches ran
+
+
 def test_riemann_quads_to_t03_parity(afar):
     """Jittered quad mesh, four-quadrant Riemann problem run to t = 0.3."""
     from afar_cudaeuler2dmixo3rk3solver_amd import cases
