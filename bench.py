@@ -327,10 +327,12 @@ def main():
 
     gpu.evolve_auto(args.warmup)
     gpu.synchronize()
+    U_t0, t_t0 = gpu.conserved(), gpu.device_time()
     wall0 = time.perf_counter()
     ms_events = gpu.time_steps(args.steps, graph=True)
     gpu.synchronize()
     wall = time.perf_counter() - wall0
+    U_t1 = gpu.conserved()
     seconds = ms_events / 1e3
     value = mesh.nc * args.steps / seconds / 1e6
     print("bench: %d steps in %.3f s (events) / %.3f s (wall)" % (args.steps, seconds, wall), file=sys.stderr,
@@ -348,6 +350,15 @@ def main():
     # standalone CFL pass, which a device-time-step RK3 step does not run).
     live_ms, live = gpu.time_steps_kernels(args.steps)
     b2b = gpu.profile(reps=20)
+    # stability: the timed run again five times from the same state (after
+    # everything above, so they change neither value nor the byte model's
+    # state); each must end in the same state bit for bit
+    reps, same = [], True
+    for _ in range(5):
+        gpu.set_conserved(U_t0)
+        gpu.set_time(t_t0)
+        reps.append(gpu.time_steps(args.steps, graph=True) / args.steps)
+        same = same and bool(np.array_equal(gpu.conserved(), U_t1))
     w = 8 if args.precision == "fp64" else 4
     ngp = 2 if args.face_quadrature else 0
     kb = kernel_bytes(st, w, ngp)
@@ -359,14 +370,27 @@ def main():
     # cfl: the standalone pass of host-requested time steps; a device-time-step
     # RK3 step gets its time step from flux_dt + dt_reduce instead (count 0)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count_per_step"])
+    # the committed counter summaries cover the default configuration only
+    default_layout = (not args.face_quadrature and args.cell_order == 0 and args.tile == 64
+                      and args.recon_lanes in (0, 4))
     d = kernels[dom]
     balg = survey_balg_per_cell_step(st, w)
+    whole_step = {"balg_bytes_per_cell_step": round(balg, 1),
+                  "balg_GBps": round(balg * value * 1e6 / 1e9, 1),
+                  "kernel_alg_bytes_per_cell_step": round(
+                      sum(kb[k] * kernels[k]["count_per_step"] for k in kb) / mesh.nc, 1)}
+    if default_layout:
+        # measured HBM bytes of one RK3 step (each kernel's committed counter
+        # summary x its launches per step) over the timed step time
+        tr = {k: measured_traffic(k, args.case, args.precision) for k in kernels if kernels[k]["count_per_step"]}
+        if all(v is not None for v in tr.values()):
+            step_bytes = sum(tr[k] * kernels[k]["count_per_step"] for k in tr)
+            gbps = step_bytes / (seconds / args.steps) / 1e9
+            whole_step.update(measured_bytes_per_step=int(step_bytes), measured_GBps=round(gbps, 1),
+                              measured_frac=round(gbps / PEAK_HBM_GBPS, 4))
     roofline = dict(bound="hbm", achieved=round(d["GBps"], 1), peak=PEAK_HBM_GBPS, unit="GB/s",
                     frac=round(d["GBps"] / PEAK_HBM_GBPS, 4),
-                    # the committed counter summaries cover the default configuration only
-                    traffic=(measured_traffic(dom, args.case, args.precision)
-                             if not args.face_quadrature and args.cell_order == 0 and args.tile == 64
-                             and args.recon_lanes in (0, 4) else None), kernel=dom,
+                    traffic=measured_traffic(dom, args.case, args.precision) if default_layout else None, kernel=dom,
                     alg_bytes_per_launch=int(d["alg_bytes"]), avg_launch_ms=round(d["avg_ms"], 5),
                     timing="hipEvents after every kernel of %d graph-replayed steps following the timed ones "
                            "(%.4f ms per step with the events)" % (args.steps, live_ms / max(args.steps, 1)))
@@ -394,10 +418,10 @@ def main():
         "cpu_baseline": cpu,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
-        "whole_step": {"balg_bytes_per_cell_step": round(balg, 1),
-                       "balg_GBps": round(balg * value * 1e6 / 1e9, 1),
-                       "kernel_alg_bytes_per_cell_step": round(
-                           sum(kb[k] * kernels[k]["count_per_step"] for k in kb) / mesh.nc, 1)},
+        "whole_step": whole_step,
+        "repeat_runs": {"ms_per_step": [round(r, 4) for r in reps], "median_ms_per_step": round(float(np.median(reps)), 4),
+                        "median_value": round(mesh.nc / (float(np.median(reps)) * 1e-3) / 1e6, 2),
+                        "same_final_state": same},
     }
     print(json.dumps(out))
 
