@@ -42,14 +42,17 @@ struct DevPtrs {
     const uint8_t* ccnt;       // nc compact stencil size
     const uint16_t* sten;      // sliced ELL, LDS-local stencil ids
     const int32_t* slot_face;  // sliced ELL, +-(face+1)
-    const double2* slot_d;     // sliced ELL, face centre - cell centre
-    const double2* slot_e;     // sliced ELL, compact neighbour - cell centre
+    const typename VT<R>::v2* slot_d;  // sliced ELL, face centre - cell centre, in the kernels' precision
+    const double2* slot_e;     // sliced ELL, compact neighbour - cell centre (K3 only)
+    R* slot_dr;                // sliced ELL, |slot_e| in the kernels' precision (written by K3)
     typename VT<R>::v2* lsq;   // sliced ELL, least-squares weights
     const R* rbf;              // sliced ELL, RBF coefficients (face slot, stencil member)
     const int32_t* halo_ptr;
     const int32_t* halo;
     const int2* fcell;         // nf own, nei
     const double4* fgeo;       // 2*nf: (Sfx, Sfy, magSf, deltaR), (dxo, dyo, dxn, dyn)
+    const float4* fgeo32;      // FP32-mixed: fgeo rounded to float (the flux pass without the CFL bound
+                               // uses the geometry in float only); else null
     typename VT<R>::v4* fv;    // 2*nf reconstructed face values, [2f + side]
     R* fvT;                    // 2*nf reconstructed face temperatures (the part the BVD test reads)
     // extension (faceQuadrature): flux at ngp points per internal face
@@ -58,7 +61,9 @@ struct DevPtrs {
     const R* rbf_gp;           // sliced ELL, RBF coefficients (point, face slot, stencil member)
     const double4* fgd;        // ngp*nf: point - owner centre (x, y), point - neighbour centre (z, w)
     typename VT<R>::v4* fvg;   // ngp*2*nf reconstructed states at the points, [(g*nf + f)*2 + side]
-    double4* flux;             // nf + nb
+    double4* flux;             // nf + nb face fluxes (FP32-mixed face-centre path: boundary faces only)
+    float4* flux32;            // FP32-mixed face-centre path: the nf internal faces' fluxes as computed,
+                               // in float (exact); else null
     const int32_t* bcell;      // nb
     const uint8_t* blabel;     // nb
     const double4* bgeo;       // nb (Sfx, Sfy, magSf, 0)

@@ -77,6 +77,8 @@ __global__ void k_lsq(DevPtrs<R> p) {
         m.x = (R)((Syy * e.x - Sxy * e.y) / (Sxx * Syy - Sxy * Sxy));
         m.y = (R)((-Sxy * e.x + Sxx * e.y) / (Sxx * Syy - Sxy * Sxy));
         p.lsq[sell(c, i, p.maxC)] = m;
+        // the centre distance K5 divides by (cudaLimiter.h:280), once here
+        p.slot_dr[sell(c, i, p.maxC)] = (R)afar_hypot(e.x, e.y);
     }
 }
 
@@ -240,8 +242,8 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
             const int isNei = sf < 0;
             const int f = (sf < 0 ? -sf : sf) - 1;
             if (f >= nf) continue;
-            const double2 d = p.slot_d[sell(c, fi, maxC)];
-            const R dx = (R)d.x, dy = (R)d.y;
+            const v2 d = p.slot_d[sell(c, fi, maxC)];
+            const R dx = d.x, dy = d.y;
             v4 o;
             o.x = wc.x + gpx * dx + gpy * dy;
             o.y = wc.y + guxx * dx + guyx * dy;
@@ -274,10 +276,9 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
         const int isNei = sf < 0;
         const int f = (sf < 0 ? -sf : sf) - 1;
         if (f >= nf) continue;
-        const double2 d = p.slot_d[sell(c, fi, maxC)];
-        const R dx = (R)d.x, dy = (R)d.y;
-        const double2 e = p.slot_e[sell(c, fi, maxC)];
-        const R deltaR = (R)afar_hypot(e.x, e.y);
+        const v2 d = p.slot_d[sell(c, fi, maxC)];
+        const R dx = d.x, dy = d.y;
+        const R deltaR = p.slot_dr[sell(c, fi, maxC)];
         const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
         const int nbl = sSt[sell(tid, fi, maxS)];
 
@@ -419,7 +420,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     int h0 = 0;
     if (t < hn) h0 = p.halo[hb + t];
     v2 Mk = v2{(R)0, (R)0};
-    double2 d = {0.0, 0.0};
+    v2 d = v2{(R)0, (R)0};
     if (inb && k < C) {
         Mk = p.lsq[sell(c, k, maxC)];
         sf = p.slot_face[sell(c, k, maxC)];
@@ -428,14 +429,14 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     v4 wHalo;
     if (t < hn) wHalo = p.W[h0];
     // A cell that took the full path at the previous launch most likely takes
-    // it again: its RBF coefficients and centre offset are requested in this
+    // it again: its RBF coefficients and centre distance are requested in this
     // round already (flat_prev only schedules loads; results do not depend on it).
     R cf[kMaxSten + 1];
-    double2 e = {0.0, 0.0};
+    R deltaR = 0;
     const R* a = p.rbf + sell(c, k * (maxS + 1), KR);  // row stride 64
     const bool early = inb && k < C && !wasFlat;
     if (early) {
-        e = p.slot_e[sell(c, k, maxC)];
+        deltaR = p.slot_dr[sell(c, k, maxC)];
 #pragma unroll
         for (int i = 0; i <= kMaxSten; ++i)
             if (i <= S) cf[i] = a[(size_t)i * 64];
@@ -489,19 +490,16 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
 
     if (k == 0) p.flat_prev[c] = flat;
     // Full path not foreseen: every RBF coefficient of the face and its centre
-    // offset are requested now, at once (one memory round trip).
+    // distance are requested now, at once (one memory round trip).
     if (!flat && live && !early) {
-        e = p.slot_e[sell(c, k, maxC)];
+        deltaR = p.slot_dr[sell(c, k, maxC)];
 #pragma unroll
         for (int i = 0; i <= kMaxSten; ++i)
             if (i <= S) cf[i] = a[(size_t)i * 64];
     }
-    const R dx = (R)d.x, dy = (R)d.y;
-    R deltaR = 0, fraction = 0;
-    if (!flat && live) {  // the offset arrives first; two values instead of four stay live
-        deltaR = (R)afar_hypot(e.x, e.y);
-        fraction = sqrt(dx * dx + dy * dy) / deltaR;
-    }
+    const R dx = d.x, dy = d.y;
+    R fraction = 0;
+    if (!flat && live) fraction = sqrt(dx * dx + dy * dy) / deltaR;
     const v4 wc = sW[lc];
 
     if (flat) {
@@ -731,8 +729,8 @@ __global__ __launch_bounds__(4 * TILE) void k_reconstruct_c(DevPtrs<R> p) {
                 p.smooth[c] = 0;
             }
             if (live) {
-                const double2 d = p.slot_d[sell(c, k, maxC)];
-                const R dx = (R)d.x, dy = (R)d.y;
+                const v2 d = p.slot_d[sell(c, k, maxC)];
+                const R dx = d.x, dy = d.y;
                 v4 o;
                 o.x = wc.x + gpx * dx + gpy * dy;
                 o.y = wc.y + guxx * dx + guyx * dy;
@@ -773,10 +771,9 @@ __global__ __launch_bounds__(4 * TILE) void k_reconstruct_c(DevPtrs<R> p) {
         const int sf = p.slot_face[sell(ci, ki, maxC)];
         const int isNei = sf < 0;
         const int f = (sf < 0 ? -sf : sf) - 1;
-        const double2 dd = p.slot_d[sell(ci, ki, maxC)];
-        const R dx = (R)dd.x, dy = (R)dd.y;
-        const double2 ee = p.slot_e[sell(ci, ki, maxC)];
-        const R deltaR = (R)afar_hypot(ee.x, ee.y);
+        const v2 dd = p.slot_d[sell(ci, ki, maxC)];
+        const R dx = dd.x, dy = dd.y;
+        const R deltaR = p.slot_dr[sell(ci, ki, maxC)];
         const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
         const int nbl = sSt[sell(lci, ki, maxS)];
         const R* a = p.rbf + sell(ci, ki * (maxS + 1), KR);
@@ -887,7 +884,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const int8_t hs0 = p.smooth[hid0];
     const int8_t hs1 = p.smooth[hid1];
     R tfo[PF], tfn[PF];
-    double dxs[PF], dys[PF];
+    R dxs[PF], dys[PF];
 #pragma unroll
     for (int fi = 0; fi < PF; ++fi) {
         const int sf = sfs[fi];
@@ -897,7 +894,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
         tfo[fi] = p.fvT[2 * fs];
         tfn[fi] = p.fvT[2 * fs + 1];
         // a slot without an internal face reads slot 0's offset (a line read anyway)
-        const double2 d = p.slot_d[sell(cc, in ? fi : 0, maxC)];
+        const v2 d = p.slot_d[sell(cc, in ? fi : 0, maxC)];
         dxs[fi] = d.x;
         dys[fi] = d.y;
     }
@@ -924,9 +921,8 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     // boundary face with bvd = 1, the largest value, so the result is the same.
     int bvd = 0, bfaces = 0;
     const R lim = to_bj(limc.w);
-    auto face = [&](int sf, R TfOwn, R TfNei, double ddx, double ddy) {
+    auto face = [&](int sf, R TfOwn, R TfNei, R dx, R dy) {
         const int isNei = sf < 0;
-        const R dx = (R)ddx, dy = (R)ddy;
         const R Tf_cur = (R)isNei * TfNei + (R)(1 - isNei) * TfOwn;
         const R Tf_nei = (R)(1 - isNei) * TfNei + (R)isNei * TfOwn;
         const R Tf_cur_TVD = Tc + lim * (gT.x * dx + gT.y * dy);
@@ -952,7 +948,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
         if (f >= nf) {
             bfaces += 1;
         } else {
-            const double2 d = p.slot_d[sell(c, fi, maxC)];
+            const v2 d = p.slot_d[sell(c, fi, maxC)];
             face(sf, p.fvT[2 * f], p.fvT[2 * f + 1], d.x, d.y);
         }
     }
@@ -1071,7 +1067,15 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
         const int f = i;
         const int2 oc = p.fcell[f];
         const int own = oc.x, nei = oc.y;
-        const double4 g0 = p.fgeo[2 * f], g1 = p.fgeo[2 * f + 1];
+        double4 g0, g1;
+        if constexpr (sizeof(R) == 4 && !DT) {  // the geometry as the float arithmetic uses it
+            const float4 h0 = p.fgeo32[2 * f], h1 = p.fgeo32[2 * f + 1];
+            g0 = double4{h0.x, h0.y, h0.z, h0.w};
+            g1 = double4{h1.x, h1.y, h1.z, h1.w};
+        } else {
+            g0 = p.fgeo[2 * f];
+            g1 = p.fgeo[2 * f + 1];
+        }
         const R dxL = (R)g1.x, dyL = (R)g1.y, dxR = (R)g1.z, dyR = (R)g1.w;
         const int8_t to = p.ftype[own], tn = p.ftype[nei];
         const int isTVD_L = to != 0, isTVD_R = tn != 0;
@@ -1137,7 +1141,10 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
         if constexpr (!GP) {
             R rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux;
             point_flux(dxL, dyL, dxR, dyR, p.fv[2 * f], p.fv[2 * f + 1], rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
-            p.flux[f] = double4{(double)rhoFlux, (double)rhoUxFlux, (double)rhoUyFlux, (double)rhoEFlux};
+            if constexpr (sizeof(R) == 4)
+                p.flux32[f] = float4{rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux};
+            else
+                p.flux[f] = double4{(double)rhoFlux, (double)rhoUxFlux, (double)rhoUyFlux, (double)rhoEFlux};
         } else {
             // extension: weighted sum over the face's quadrature points
             double4 acc = double4{0.0, 0.0, 0.0, 0.0};
@@ -1191,6 +1198,15 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
 // (the residual gathered as in the update), then one block adds the partials
 // in index order.  Fixed launch shapes make the sum order, and so the value,
 // reproducible.
+// The flux of face f (internal or boundary) as the residual gather reads it.
+template <typename R> __device__ __forceinline__ double4 face_flux(const DevPtrs<R>& p, int f) {
+    if (p.flux32 && f < p.nf) {
+        const float4 q = p.flux32[f];
+        return double4{(double)q.x, (double)q.y, (double)q.z, (double)q.w};
+    }
+    return p.flux[f];
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_residual_part(DevPtrs<R> p) {
     const int c = blockIdx.x * 256 + threadIdx.x;
@@ -1202,7 +1218,7 @@ __global__ __launch_bounds__(256) void k_residual_part(DevPtrs<R> p) {
             const int sf = p.slot_face[sell(c, fi, maxC)];
             const int dir = (sf > 0) - (sf < 0);
             const int f = (sf < 0 ? -sf : sf) - 1;
-            r0 += dir * p.flux[f].x;
+            r0 += dir * face_flux(p, f).x;
         }
         const double q = r0 / p.vol[c];
         v = q * q;
@@ -1299,7 +1315,7 @@ __global__ __launch_bounds__(256) void k_update(DevPtrs<R> p, double Rgas, doubl
         const int sf = p.slot_face[sell(c, fi, maxC)];
         const int dir = (sf > 0) - (sf < 0);
         const int f = (sf < 0 ? -sf : sf) - 1;
-        const double4 F = p.flux[f];
+        const double4 F = face_flux(p, f);
         r0 += dir * F.x;
         r1 += dir * F.y;
         r2 += dir * F.z;

@@ -412,6 +412,15 @@ class SolverImpl final : public DeviceSolver {
         if (!src) throw Error(1, std::string("unknown device array ") + what);
         if (bytes != have) throw Error(1, std::string("size mismatch fetching ") + what);
         synchronize();
+        if (P_.flux32 && strcmp(what, "flux") == 0) {  // internal faces from their float store
+            std::vector<float4> q(nf);
+            if (nf) check_hip(hipMemcpy(q.data(), P_.flux32, sizeof(float4) * nf, hipMemcpyDeviceToHost), "debug copy");
+            double4* o = static_cast<double4*>(out);
+            for (size_t f = 0; f < nf; ++f) o[f] = double4{q[f].x, q[f].y, q[f].z, q[f].w};
+            if (nb)
+                check_hip(hipMemcpy(o + nf, P_.flux + nf, sizeof(double4) * nb, hipMemcpyDeviceToHost), "debug copy");
+            return;
+        }
         check_hip(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost), "debug copy");
     }
 
@@ -539,8 +548,16 @@ class SolverImpl final : public DeviceSolver {
         P_.ccnt = put(m_ccnt, L_.ccnt.data(), nc);
         P_.sten = put(m_sten, L_.sten.data(), L_.sten.size());
         P_.slot_face = put(m_slot_face, L_.slot_face.data(), nslot);
-        P_.slot_d = reinterpret_cast<const double2*>(put(m_slot_d, L_.slot_d.data(), 2 * nslot));
+        {
+            // face offsets in the kernels' precision (they use them in it only)
+            std::vector<R> d(2 * nslot);
+            for (size_t i = 0; i < d.size(); ++i) d[i] = (R)L_.slot_d[i];
+            P_.slot_d = reinterpret_cast<const v2*>(put(m_slot_d, d.data(), d.size()));
+        }
         P_.slot_e = reinterpret_cast<const double2*>(put(m_slot_e, L_.slot_e.data(), 2 * nslot));
+        m_slot_dr.alloc(sizeof(R) * nslot);
+        P_.slot_dr = m_slot_dr.as<R>();
+        check_hip(hipMemset(P_.slot_dr, 0, sizeof(R) * nslot), "hipMemset");
         m_lsq.alloc(sizeof(v2) * nslot);
         P_.lsq = m_lsq.as<v2>();
         {
@@ -558,6 +575,12 @@ class SolverImpl final : public DeviceSolver {
         }
         P_.fcell = reinterpret_cast<const int2*>(put(m_fcell, L_.fcell.data(), 2 * nf));
         P_.fgeo = reinterpret_cast<const double4*>(put(m_fgeo, L_.fgeo.data(), 8 * nf));
+        P_.fgeo32 = nullptr;
+        if (sizeof(R) == 4) {
+            std::vector<float> g(L_.fgeo.size());
+            for (size_t i = 0; i < g.size(); ++i) g[i] = (float)L_.fgeo[i];
+            P_.fgeo32 = reinterpret_cast<const float4*>(put(m_fgeo32, g.data(), g.size()));
+        }
         m_fv.alloc(sizeof(v4) * 2 * nf);
         P_.fv = m_fv.as<v4>();
         m_fvT.alloc(sizeof(R) * 2 * nf);
@@ -575,6 +598,11 @@ class SolverImpl final : public DeviceSolver {
         }
         m_flux.alloc(sizeof(double4) * (nf + nb));
         P_.flux = m_flux.as<double4>();
+        P_.flux32 = nullptr;
+        if (sizeof(R) == 4 && L_.ngp == 0) {  // face-centre fluxes are floats: stored as such
+            m_flux32.alloc(sizeof(float4) * (nf ? nf : 1));
+            P_.flux32 = m_flux32.as<float4>();
+        }
         P_.bcell = put(m_bcell, L_.bcell.data(), nb);
         P_.blabel = put(m_blabel, L_.blabel.data(), nb);
         P_.bgeo = reinterpret_cast<const double4*>(put(m_bgeo, L_.bgeo.data(), 4 * nb));
@@ -631,8 +659,9 @@ class SolverImpl final : public DeviceSolver {
     size_t lds_recon_ = 0, lds_bvd_ = 0, lds_m_off_ = 0, lds_bvd_sten_off_ = 0;
     int lanes_ = 1;  // threads per cell of the reconstruction kernel
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
-        m_slot_face, m_slot_d, m_slot_e, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fv, m_flux, m_bcell,
-        m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res, m_rbf_gp, m_fgd, m_fvg, m_flat_prev;
+        m_slot_face, m_slot_d, m_slot_e, m_slot_dr, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fgeo32,
+        m_fv, m_flux, m_flux32, m_bcell, m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res,
+        m_rbf_gp, m_fgd, m_fvg, m_flat_prev;
 };
 
 // A solver on relabelled cells and faces (compact_tile_order) that reads and

@@ -62,27 +62,32 @@ def kernel_bytes(st, w=8, ngp=0):
     """Compulsory bytes per launch of each kernel (every array element touched
     once; neighbour re-reads counted as cache hits).  w = width of the
     reconstruction/flux type (8 FP64, 4 FP32-mixed); state stays FP64.  The
-    reconstruction reads the RBF coefficients and the neighbour-centre offsets
-    (slot_e) only for cells on the full path.  ngp > 0 (flux quadrature
+    reconstruction reads the RBF coefficients and the neighbour-centre
+    distances (slot_dr) only for cells on the full path; face offsets
+    (slot_d) and distances are stored in the kernels' precision.  ngp > 0 (flux quadrature
     points): the reconstruction also reads each point's coefficients (full
     path) and offsets and writes each point's states; the flux reads those
     states and offsets instead of the face-centre states."""
     nc, nf, nb, sS, sC = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"]
     sR, sfull = st["sum_int_coef_full"], st["int_slots_full"]
     ns = 2 * nf  # internal face slots
-    recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 16 * ns + 16 * sfull + 2 * w * sC + w * sR
+    recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * sfull + 2 * w * sC + w * sR
              + 8 * w * nc + 4 * w * nc + nc + 4 * w * ns + w * ns)
-    bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + 16 * ns + w * ns + 4 * w * nc + 2 * w * nc + w * nc
+    bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * ns + 4 * w * nc + 2 * w * nc + w * nc
            + nc + 4 * w * nc)
-    flux = 8 * nf + 64 * nf + 8 * w * nf + nc * (4 * w + 4 * w + 8 * w + 1) + 32 * nb + 4 * w * nb + 32 * (nf + nb)
-    update = nc + 4 * sC + 32 * (nf + nb) + 32 * nc * 2 + 8 * nc + 65 * nb + 32 * nc + 4 * w * (nc + nb) + 32 * nc / 3
+    # FP32-mixed: internal fluxes at the face centre are stored as floats, and
+    # the flux pass without the CFL bound reads the face geometry in float
+    fw = 4 * w if (w == 4 and not ngp) else 32
+    geo = 8 * w
+    flux = 8 * nf + geo * nf + 8 * w * nf + nc * (4 * w + 4 * w + 8 * w + 1) + 32 * nb + 4 * w * nb + fw * nf + 32 * nb
+    update = nc + 4 * sC + fw * nf + 32 * nb + 32 * nc * 2 + 8 * nc + 65 * nb + 32 * nc + 4 * w * (nc + nb) + 32 * nc / 3
     cfl = 8 * nf + 48 * nf + 4 * w * nc
     if ngp:
         recon += ngp * (w * sR + 4 * w * ns + 32 * ns)
         flux += ngp * (8 * w + 32) * nf - 8 * w * nf
     nblk = 8 * (((nf + nb + 255) // 256 + 7) // 8)  # flux grid: one CFL partial per block
-    return dict(cfl=cfl, reconstruct=recon, bvd=bvd, flux=flux, update=update, flux_dt=flux + 8 * nblk,
-                dt_reduce=8 * nblk)
+    return dict(cfl=cfl, reconstruct=recon, bvd=bvd, flux=flux, update=update,
+                flux_dt=flux + (64 - geo) * nf + 8 * nblk, dt_reduce=8 * nblk)
 
 
 def survey_balg_per_cell_step(st, w=8):
