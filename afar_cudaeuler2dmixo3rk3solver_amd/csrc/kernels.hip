@@ -1322,7 +1322,10 @@ __global__ __launch_bounds__(256) void k_update(DevPtrs<R> p, double Rgas, doubl
         r3 += dir * F.w;
     }
     const double V = p.vol[c];
-    const double4 uo = p.Uold[c], u = p.U[c];
+    // b2 = 0 (first stage): U equals Uold at the start of every step (both
+    // are stored from one value at the end of a step and at upload), so the
+    // stage reads it once; b2 * u is then b2 * uo, the same value
+    const double4 uo = p.Uold[c], u = b2 == 0.0 ? uo : p.U[c];
     const double rho = b1 * uo.x + b2 * u.x + b3 * dt * (-r0 / V);
     const double rhoUx = b1 * uo.y + b2 * u.y + b3 * dt * (-r1 / V);
     const double rhoUy = b1 * uo.z + b2 * u.z + b3 * dt * (-r2 / V);
