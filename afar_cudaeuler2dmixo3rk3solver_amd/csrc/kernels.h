@@ -59,7 +59,8 @@ struct DevPtrs {
     int ngp;                   // 0: face centre only (reference)
     double gpw[4];             // point weights, sum 1
     const R* rbf_gp;           // sliced ELL, RBF coefficients (point, face slot, stencil member)
-    const double4* fgd;        // ngp*nf: point - owner centre (x, y), point - neighbour centre (z, w)
+    const typename VT<R>::v4* fgd;  // ngp*nf: point - owner centre (x, y), point - neighbour centre (z, w),
+                                    // in the kernels' precision
     typename VT<R>::v4* fvg;   // ngp*2*nf reconstructed states at the points, [(g*nf + f)*2 + side]
     double4* flux;             // nf + nb face fluxes (FP32-mixed face-centre path: boundary faces only)
     float4* flux32;            // FP32-mixed face-centre path: the nf internal faces' fluxes as computed,

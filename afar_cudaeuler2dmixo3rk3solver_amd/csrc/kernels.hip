@@ -253,7 +253,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
             p.fvT[2 * f + isNei] = o.w;
             if constexpr (GP)
                 for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
-                const double4 dg = p.fgd[(size_t)g * nf + f];
+                const v4 dg = p.fgd[(size_t)g * nf + f];
                 const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
                 v4 og;
                 og.x = wc.x + gpx * gx + gpy * gy;
@@ -322,7 +322,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
                 uyg += cg * wj.z;
                 Tg += cg * wj.w;
             }
-            const double4 dg = p.fgd[(size_t)g * nf + f];
+            const v4 dg = p.fgd[(size_t)g * nf + f];
             const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
             v4 og;
             og.x = (R)(rP >= K<R>::SMALL) * pg + (R)(rP < K<R>::SMALL) * (wc.x + (gpx * gx + gpy * gy));
@@ -521,7 +521,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
             p.fvT[2 * f + isNei] = o.w;
             if constexpr (GP)
                 for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
-                const double4 dg = p.fgd[(size_t)g * nf + f];
+                const v4 dg = p.fgd[(size_t)g * nf + f];
                 const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
                 v4 og;
                 og.x = wc.x + gpx * gx + gpy * gy;
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
                 uyg += cg * wj.z;
                 Tg += cg * wj.w;
             }
-            const double4 dg = p.fgd[(size_t)g * nf + f];
+            const v4 dg = p.fgd[(size_t)g * nf + f];
             const R gx = (R)(isNei ? dg.z : dg.x), gy = (R)(isNei ? dg.w : dg.y);
             v4 og;
             og.x = (R)(rP >= K<R>::SMALL) * pg + (R)(rP < K<R>::SMALL) * (wc.x + (gpx * gx + gpy * gy));
@@ -1149,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
             // extension: weighted sum over the face's quadrature points
             double4 acc = double4{0.0, 0.0, 0.0, 0.0};
             for (int g = 0; g < p.ngp; ++g) {
-                const double4 d = p.fgd[(size_t)g * nf + f];
+                const v4 d = p.fgd[(size_t)g * nf + f];
                 R rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux;
                 point_flux((R)d.x, (R)d.y, (R)d.z, (R)d.w, p.fvg[((size_t)g * nf + f) * 2],
                            p.fvg[((size_t)g * nf + f) * 2 + 1], rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);

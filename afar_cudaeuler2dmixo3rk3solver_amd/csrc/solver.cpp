@@ -592,7 +592,9 @@ class SolverImpl final : public DeviceSolver {
             std::vector<R> rg(L_.rbf_gp.size());
             for (size_t i = 0; i < rg.size(); ++i) rg[i] = (R)L_.rbf_gp[i];
             P_.rbf_gp = put(m_rbf_gp, rg.data(), rg.size());
-            P_.fgd = reinterpret_cast<const double4*>(put(m_fgd, L_.fgd.data(), L_.fgd.size()));
+            std::vector<R> gd(L_.fgd.size());
+            for (size_t i = 0; i < gd.size(); ++i) gd[i] = (R)L_.fgd[i];
+            P_.fgd = reinterpret_cast<const v4*>(put(m_fgd, gd.data(), gd.size()));
             m_fvg.alloc(sizeof(v4) * 2 * nf * L_.ngp);
             P_.fvg = m_fvg.as<v4>();
         }

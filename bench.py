@@ -84,8 +84,8 @@ def kernel_bytes(st, w=8, ngp=0):
     update = nc + 4 * sC + fw * nf + 32 * nb + 32 * nc * 5 / 3 + 8 * nc + 65 * nb + 32 * nc + 4 * w * (nc + nb) + 32 * nc / 3
     cfl = 8 * nf + 48 * nf + 4 * w * nc
     if ngp:
-        recon += ngp * (w * sR + 4 * w * ns + 32 * ns)
-        flux += ngp * (8 * w + 32) * nf - 8 * w * nf
+        recon += ngp * (w * sR + 4 * w * ns + 4 * w * ns)
+        flux += ngp * (8 * w + 4 * w) * nf - 8 * w * nf
     nblk = 8 * (((nf + nb + 255) // 256 + 7) // 8)  # flux grid: one CFL partial per block
     return dict(cfl=cfl, reconstruct=recon, bvd=bvd, flux=flux, update=update,
                 flux_dt=flux + (64 - geo) * nf + 8 * nblk, dt_reduce=8 * nblk)
