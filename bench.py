@@ -267,7 +267,8 @@ def main():
     if args.hbm_table:
         return hbm_table(*args.hbm_table, copy_mib=args.copy_calibration or 1024)
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding); "
+        print("bench.py runs on one GPU only: there is no multi-GPU path (meshes would shard by cell ranges "
+              "with halo exchanges, not implemented); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.bandwidth_probe:
