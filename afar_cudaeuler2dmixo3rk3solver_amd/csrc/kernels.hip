@@ -957,16 +957,23 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     if (p.boundaryFirstOrder && bfaces > 0) {
         // extension: a cell with a boundary face takes the first-order state
         // on its faces (TVD with zero limiter), generalising the reference's
-        // corner-cell rule; shock types keep their flux choice
+        // corner-cell rule; shock types keep their flux choice.  A raw
+        // limiter is never negative, so -1 marks the zero limiter.
         finalType = finalType ? finalType : 1;
         p.ftype[c] = (int8_t)finalType;
-        p.lim[c] = v4{(R)0, (R)0, (R)0, (R)0};
+        p.lim[c] = v4{(R)-1, (R)-1, (R)-1, (R)-1};
         return;
     }
+    // The limiter transform (cudaLimiter.h:453-461) is applied where the
+    // limiter is used, by the face pass, from this type and the raw limiter:
+    // the same function of the same values, without a write of the array.
     p.ftype[c] = (int8_t)finalType;
-    const int isShock = finalType < 0;
-    p.lim[c] = v4{transform_limiter(limc.x, isShock), transform_limiter(limc.y, isShock),
-                  transform_limiter(limc.z, isShock), transform_limiter(limc.w, isShock)};
+}
+
+// The limiter the face pass uses: the transform of K6 applied to the raw K5
+// limiter with the cell's final type, or zero for a first-order cell.
+template <typename R> __device__ __forceinline__ R face_limiter(R raw, int isShock) {
+    return raw < (R)0 ? (R)0 : transform_limiter(raw, isShock);
 }
 
 // ---------------------------------------------------------------- K7 flux
@@ -1083,7 +1090,12 @@ __global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double 
         const R magSf = (R)g0.z, Sfx = (R)g0.x, Sfy = (R)g0.y;
         const R Rg = (R)Rgas;
         const v4 wL = p.W[own], wR = p.W[nei];
-        const v4 lL = p.lim[own], lR = p.lim[nei];
+        const v4 rL = p.lim[own], rR = p.lim[nei];
+        const int shL = to < 0, shR = tn < 0;
+        const v4 lL = v4{face_limiter(rL.x, shL), face_limiter(rL.y, shL), face_limiter(rL.z, shL),
+                         face_limiter(rL.w, shL)};
+        const v4 lR = v4{face_limiter(rR.x, shR), face_limiter(rR.y, shR), face_limiter(rR.z, shR),
+                         face_limiter(rR.w, shR)};
         const v2 gpL = p.gp[own], guL = p.gu[own], gvL = p.gv[own], gTL = p.gT[own];
         const v2 gpR = p.gp[nei], guR = p.gu[nei], gvR = p.gv[nei], gTR = p.gT[nei];
         if (DT) {

@@ -73,8 +73,8 @@ def kernel_bytes(st, w=8, ngp=0):
     ns = 2 * nf  # internal face slots
     recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * sfull + 2 * w * sC + w * sR
              + 8 * w * nc + 4 * w * nc + nc + 4 * w * ns + w * ns)
-    bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * ns + 4 * w * nc + 2 * w * nc + w * nc
-           + nc + 4 * w * nc)
+    # the limiter transform is applied by the face pass: bvd writes only the types
+    bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * ns + 4 * w * nc + 2 * w * nc + w * nc + nc)
     # FP32-mixed: internal fluxes at the face centre are stored as floats, and
     # the flux pass without the CFL bound reads the face geometry in float
     fw = 4 * w if (w == 4 and not ngp) else 32
