@@ -237,7 +237,10 @@ int afar_solver_profile(afar_solver* h, int32_t reps, double* ms, int32_t* count
 /* Device sizes: out[0..7] = nc, nf, nb, maxS, maxC, tile, ntiles, maxHalo. */
 int afar_solver_sizes(afar_solver* h, int64_t* out);
 /* Test hooks: run stage kernels (0 reconstruct, 1 +bvd, 2 +flux, 3 +update of
- * RK stage `stage` with step dt) and copy named device arrays (mesh order). */
+ * RK stage `stage` with step dt) and copy named device arrays (mesh order):
+ * W, U, gp, gu, gv, gT, lim (the raw K5 limiter: the flux pass applies K6's
+ * transform as it reads it; -1 marks a first-order cell), smooth, ftype, fv,
+ * fvg, flux (FP64 for every face in both precisions), lsq. */
 int afar_solver_run_stage(afar_solver* h, int32_t upto, double dt, int32_t stage);
 int afar_solver_fetch(afar_solver* h, const char* name, void* out, size_t bytes);
 void afar_solver_destroy(afar_solver* h);
