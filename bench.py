@@ -430,6 +430,7 @@ def main():
                         "median_value": round(mesh.nc / (float(np.median(reps)) * 1e-3) / 1e6, 2),
                         "same_final_state": same},
     }
+    out["command"] = " ".join(["python", "bench.py"] + sys.argv[1:])
     print(json.dumps(out))
 
 
@@ -499,6 +500,7 @@ def run_batch(args):
         "roofline": None, "cpu_baseline": None,
     }
     bb.close()
+    out["command"] = " ".join(["python", "bench.py"] + sys.argv[1:])
     print(json.dumps(out))
 
 
