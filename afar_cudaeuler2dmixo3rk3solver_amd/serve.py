@@ -308,8 +308,11 @@ def main(argv=None):
     done, wall = srv.run(reqs)
     tts = np.array([r.times["finish"] - r.arrival for r in done])
     cell_steps = float(sum(r.cells * r.steps for r in done))
+    # a seeded sample of ceil(fraction x requests) requests, at least one
     rng = np.random.default_rng(args.seed + 1)
-    sample = [r for r in done if rng.uniform() < args.sample]
+    k = min(len(done), max(1, int(np.ceil(args.sample * len(done)))))
+    order = sorted(done, key=lambda r: r.rid)
+    sample = [order[i] for i in sorted(rng.choice(len(order), size=k, replace=False))] if done else []
     errs = []
     for r in sample:
         ref = fp64_reference(r, args.chunk)
