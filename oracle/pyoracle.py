@@ -39,34 +39,39 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def load(path):
+    """The oracle library at `path` with its ctypes signatures."""
+    L = C.CDLL(path)
+    pf = P(OracleFields)
+    d = C.c_double
+    L.oracle_least_square_matrix.argtypes = [pf]
+    L.oracle_adjust_time_step.argtypes = [pf, d, d, d]
+    L.oracle_adjust_time_step.restype = d
+    L.oracle_reconstruct.argtypes = [pf]
+    L.oracle_bvd.argtypes = [pf]
+    L.oracle_evaluate_flux.argtypes = [pf, d, d]
+    L.oracle_evaluate_residual.argtypes = [pf]
+    L.oracle_update.argtypes = [pf, d, d, d, d, d, d]
+    L.oracle_update_boundary.argtypes = [pf, d, d]
+    L.oracle_store_conserved.argtypes = [pf]
+    L.oracle_evolve.argtypes = [pf, d, d, d, C.c_int]
+    L.oracle_run.argtypes = [pf, d, d, d, C.c_int, C.c_int, P(d)]
+    L.oracle_roe_flux.argtypes = [P(d)] + [d] * 17
+    L.oracle_kfvs_flux.argtypes = [P(d)] + [d] * 15
+    L.oracle_central_flux.argtypes = [P(d)] + [d] * 13
+    L.oracle_exp.argtypes = [d]
+    L.oracle_exp.restype = d
+    L.oracle_erfc.argtypes = [d]
+    L.oracle_erfc.restype = d
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        pf = P(OracleFields)
-        d = C.c_double
-        L.oracle_least_square_matrix.argtypes = [pf]
-        L.oracle_adjust_time_step.argtypes = [pf, d, d, d]
-        L.oracle_adjust_time_step.restype = d
-        L.oracle_reconstruct.argtypes = [pf]
-        L.oracle_bvd.argtypes = [pf]
-        L.oracle_evaluate_flux.argtypes = [pf, d, d]
-        L.oracle_evaluate_residual.argtypes = [pf]
-        L.oracle_update.argtypes = [pf, d, d, d, d, d, d]
-        L.oracle_update_boundary.argtypes = [pf, d, d]
-        L.oracle_store_conserved.argtypes = [pf]
-        L.oracle_evolve.argtypes = [pf, d, d, d, C.c_int]
-        L.oracle_run.argtypes = [pf, d, d, d, C.c_int, C.c_int, P(d)]
-        L.oracle_roe_flux.argtypes = [P(d)] + [d] * 17
-        L.oracle_kfvs_flux.argtypes = [P(d)] + [d] * 15
-        L.oracle_central_flux.argtypes = [P(d)] + [d] * 13
-        L.oracle_exp.argtypes = [d]
-        L.oracle_exp.restype = d
-        L.oracle_erfc.argtypes = [d]
-        L.oracle_erfc.restype = d
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
 
 
