@@ -109,6 +109,7 @@ SIGNATURES = {
     "afar_pinv": ([C.c_int32, _D, _D], C.c_int),
     "afar_mesh_export_reference": ([_VP, P(Schemes), _D, P(FieldPointer)], C.c_int),
     "afar_solver_create": ([_VP, P(Schemes), _D, P(_VP)], C.c_int),
+    "afar_solver_create_on_stream": ([_VP, P(Schemes), _D, _VP, P(_VP)], C.c_int),
     "afar_solver_set_delta_t": ([_VP, _D], C.c_int),
     "afar_solver_evolve": ([_VP, C.c_double], C.c_int),
     "afar_solver_evolve_auto": ([_VP, C.c_int32], C.c_int),
@@ -142,15 +143,16 @@ SIGNATURES = {
     "afar_batch_write_field": ([_VP, C.c_int32, _D, _I8], C.c_int),
     "afar_batch_profile": ([_VP, C.c_int32, _D, _I32], C.c_int),
     "afar_batch_destroy": ([_VP], None),
-    "afar_set_device_field_data": ([P(_VP), P(FieldPointer), C.c_int32, C.c_int32, C.c_int32], C.c_int),
+    "afar_set_device_field_data": ([P(_VP), P(FieldPointer), C.c_int64, C.c_int64, C.c_int64], C.c_int),
     "afar_device_configure": ([_VP, C.c_int32, C.c_int32], C.c_int),
-    "afar_set_device_stencil_data": ([_VP, P(FieldPointer), C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                                      C.c_int32, C.c_int32, C.c_int32], C.c_int),
+    "afar_device_set_stream": ([_VP, _VP], C.c_int),
+    "afar_set_device_stencil_data": ([_VP, P(FieldPointer), C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                                      C.c_int64, C.c_int64, C.c_int64], C.c_int),
     "afar_free_field_data": ([_VP], C.c_int),
-    "afar_adjust_time_step": ([_VP, C.c_int32, C.c_double, C.c_double, C.c_double, _D], C.c_int),
-    "afar_gpu_evolve": ([_VP, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                         C.c_int32, C.c_int32, C.c_int32], C.c_int),
-    "afar_copy_field_data_device_to_host": ([P(FieldPointer), _VP, C.c_int32, C.c_int32], C.c_int),
+    "afar_adjust_time_step": ([_VP, C.c_int64, C.c_double, C.c_double, C.c_double, _D], C.c_int),
+    "afar_gpu_evolve": ([_VP, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+                         C.c_int64, C.c_int64, C.c_int64], C.c_int),
+    "afar_copy_field_data_device_to_host": ([P(FieldPointer), _VP, C.c_int64, C.c_int64], C.c_int),
     "afar_selftest_math": ([_D, _D, C.c_int32], C.c_int),
     "afar_host_math": ([_D, _D, C.c_int32], C.c_int),
     "afar_device_info": ([C.c_char_p, C.c_int32, _I32, _I64], C.c_int),

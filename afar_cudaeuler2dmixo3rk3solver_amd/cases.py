@@ -147,3 +147,127 @@ def vortex_case(n=64, seed=0, jitter=0.15, L=10.0, R=R_REF, averaged=False, quad
         return mesh, W
     Ubar = cell_average(mesh, lambda x, y: vortex_conserved(x, y, L=L), n=quad)
     return mesh, primitive_from_conserved(Ubar, R), Ubar
+
+
+# ---------------------------------------------------------------- the reference's example case
+
+# inflow state of the cornerFlow case's inlet patch (p, Ux, Uy, T): the fixedValue of its 0/ fields,
+# which is also the value of boundary label 5 (src/CUDA/BCs/2D/fixedValue2Boundary.h)
+CORNER_FLOW_INLET = (30.05945, 4.07795, 0.0, 5.97677)
+
+
+def corner_flow_mesh(nx=500, ny=250, renumber=True):
+    """The geometry of the reference's example case
+    (simulation/cornerFlow/cornerFlow_cuda/system/blockMeshDict:18-75): the
+    unit square as two blocks of nx x ny quads, y in [0, 0.5] and [0.5, 1]
+    (the reference has nx = 500, ny = 250).  Patches in the case's order:
+    0 inlet (x = 0, y > 0.5; label 5), 1 obstacle (x = 0, y < 0.5, and the
+    walls y = 0 and y = 1; label 2), 2 outlet (x = 1; label 4).  Cells are
+    RCM-renumbered, as the case's `renumberMesh -overwrite` does."""
+    nyt = 2 * ny
+    xs = np.linspace(0.0, 1.0, nx + 1)
+    ys = np.concatenate([np.linspace(0.0, 0.5, ny + 1), np.linspace(0.5, 1.0, ny + 1)[1:]])
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel()], axis=1)
+
+    def P(i, j):
+        return i * (nyt + 1) + j
+
+    I, J = np.meshgrid(np.arange(nx), np.arange(nyt), indexing="ij")
+    I, J = I.T.ravel(), J.T.ravel()  # blockMesh order: x fastest, block 0 (lower) first
+    quads = np.stack([P(I, J), P(I + 1, J), P(I + 1, J + 1), P(I, J + 1)], axis=1)
+    cell_ptr = np.arange(0, 4 * len(quads) + 1, 4)
+    bedges, bpatch = [], []
+    for j in range(nyt):
+        bedges.append((P(0, j), P(0, j + 1)))
+        bpatch.append(0 if j >= ny else 1)
+        bedges.append((P(nx, j), P(nx, j + 1)))
+        bpatch.append(2)
+    for i in range(nx):
+        bedges.append((P(i, 0), P(i + 1, 0)))
+        bpatch.append(1)
+        bedges.append((P(i, nyt), P(i + 1, nyt)))
+        bpatch.append(1)
+    return Mesh.from_polygons(pts, cell_ptr, quads.ravel(), bedges, bpatch, [5, 2, 4], renumber=renumber)
+
+
+def _foam_header(cls, obj):
+    return ("FoamFile\n{\n    version     2.0;\n    format      ascii;\n    class       %s;\n"
+            "    object      %s;\n}\n\n" % (cls, obj))
+
+
+def write_corner_flow_case(case_dir, nx=500, ny=250, end_time=0.15, write_interval=0.01, max_co=0.3,
+                           c_order=7, f_order=5, R=R_REF, Cv=CV_REF):
+    """A complete case directory with the settings of the reference's
+    cornerFlow case: constant/polyMesh of corner_flow_mesh(nx, ny) (what
+    blockMesh and renumberMesh give), 0/{p,U,T} (quiescent p = 1, U = 0,
+    T = 1; inlet fixedValue CORNER_FLOW_INLET, obstacle zeroGradient / slip,
+    outlet zeroGradient; 0/p:19-35, 0/U:19-35, 0/T:19-35 of the case),
+    system/controlDict (adjustable time step with maxCo, endTime, writes
+    every write_interval; controlDict:25-51) and system/fvSchemes
+    (gaussQuadrature cOrder/fOrder; CUDA R, Cv, edgeNeighbour, recalcBasis,
+    boundary patch0 5, patch1 2, patch2 4; fvSchemes:17-36).  Returns the
+    mesh (mesh order = file order)."""
+    import os
+    mesh = corner_flow_mesh(nx, ny)
+    mesh.write_foam(case_dir)
+    names = [n for n, _ in mesh.patches()]
+    os.makedirs(os.path.join(case_dir, "0"), exist_ok=True)
+    os.makedirs(os.path.join(case_dir, "system"), exist_ok=True)
+    inlet, obstacle, outlet = names[:3]
+    p_in, ux_in, uy_in, T_in = CORNER_FLOW_INLET
+
+    def field(cls, obj, dims, internal, inlet_value, obstacle_type):
+        return (_foam_header(cls, obj) + "dimensions      %s;\n\ninternalField   %s;\n\nboundaryField\n{\n"
+                "    %s\n    {\n        type            fixedValue;\n        value           %s;\n    }\n"
+                "    %s\n    {\n        type            %s;\n    }\n"
+                "    %s\n    {\n        type            zeroGradient;\n    }\n"
+                "    frontAndBack\n    {\n        type            empty;\n    }\n}\n"
+                % (dims, internal, inlet, inlet_value, obstacle, obstacle_type, outlet))
+
+    with open(os.path.join(case_dir, "0", "p"), "w") as f:
+        f.write(field("volScalarField", "p", "[1 -1 -2 0 0 0 0]", "uniform 1", "uniform %.17g" % p_in,
+                      "zeroGradient"))
+    with open(os.path.join(case_dir, "0", "U"), "w") as f:
+        f.write(field("volVectorField", "U", "[0 1 -1 0 0 0 0]", "uniform (0 0 0)",
+                      "uniform (%.17g %.17g 0)" % (ux_in, uy_in), "slip"))
+    with open(os.path.join(case_dir, "0", "T"), "w") as f:
+        f.write(field("volScalarField", "T", "[0 0 0 1 0 0 0]", "uniform 1", "uniform %.17g" % T_in,
+                      "zeroGradient"))
+    with open(os.path.join(case_dir, "system", "controlDict"), "w") as f:
+        f.write(_foam_header("dictionary", "controlDict"))
+        f.write("application     cudaEuler2DmixO3RK3solver;\nstartFrom       startTime;\nstartTime       0;\n"
+                "stopAt          endTime;\nendTime         %r;\ndeltaT          1e-05;\n"
+                "writeControl    adjustableRunTime;\nwriteInterval   %r;\nadjustTimeStep  yes;\n"
+                "maxCo           %r;\n" % (float(end_time), float(write_interval), float(max_co)))
+    with open(os.path.join(case_dir, "system", "fvSchemes"), "w") as f:
+        f.write(_foam_header("dictionary", "fvSchemes"))
+        f.write("gaussQuadrature\n{\n    cOrder %d;\n    fOrder %d;\n}\n\nCUDA\n{\n    R  %r;\n    Cv  %r;\n"
+                "    stencilType  edgeNeighbour;\n    recalcBasis  true;\n\n    boundary\n    {\n"
+                "        patch0  5;\n        patch1  2;\n        patch2  4;\n    }\n}\n" % (c_order, f_order, R, Cv))
+    return mesh
+
+
+def foam_time_loop(end_time, write_interval, step):
+    """The application's time loop (cudaEuler2DmixO3RK3solver.cpp:53-79 with
+    Foam::Time's adjustableRunTime writes): step(t) -> CFL time step of the
+    current state, clipped to the next write time and to the end time; then
+    the time advances and `yield t, dt, write_now` (the caller evolves by dt)."""
+    t, nxt = 0.0, write_interval
+    tol = 1e-12 * write_interval
+    while t < end_time - 1e-12 * (end_time if end_time > 0 else 1.0):
+        d = step(t)
+        to_write, to_end = nxt - t, end_time - t
+        if to_write > 0 and d > to_write:
+            d = to_write
+        if to_end > 0 and d > to_end:
+            d = to_end
+        t += d
+        write_now = False
+        if t >= nxt - tol:
+            write_now = True
+            while nxt <= t + tol:
+                nxt += write_interval
+        if t >= end_time - tol:
+            write_now = True
+        yield t, d, write_now

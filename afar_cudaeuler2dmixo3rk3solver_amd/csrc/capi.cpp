@@ -351,7 +351,10 @@ int afar_mesh_precompute_gpu(afar_mesh* h, const afar_schemes* s) {
         Schemes st_only = sc;
         st_only.recalcBasis = 0;  // stencils on the host, coefficients on the GPU
         Precomputed pc = precompute(h->m, st_only);
-        precompute_rbf_gpu(h->m, sc, pc);
+        {
+            std::lock_guard<std::mutex> lk(runtime_mutex());  // device allocations and frees
+            precompute_rbf_gpu(h->m, sc, pc);
+        }
         h->pc = std::move(pc);
         finish_precompute(h, sc);
     });
@@ -551,11 +554,13 @@ int afar_mesh_export_reference(const afar_mesh* h, const afar_schemes* s, const 
 
 // ---------------------------------------------------------------- seam 1
 
-int afar_solver_create(const afar_mesh* h, const afar_schemes* s, const double* W0, afar_solver** out) {
+static int solver_create(const afar_mesh* h, const afar_schemes* s, const double* W0, hipStream_t stream,
+                         afar_solver** out) {
     return guard([&] {
         need(out != nullptr, "out is NULL");
         RefArrays ra = ref_arrays_of(h, s, W0);
         SolverConfig cfg;
+        cfg.stream = stream;
         cfg.R = s->R;
         cfg.Cv = s->Cv;
         cfg.CFL = s->maxCo;
@@ -569,12 +574,22 @@ int afar_solver_create(const afar_mesh* h, const afar_schemes* s, const double* 
         need(s->cellOrder >= 0 && s->cellOrder <= 2, "cellOrder must be 0, 1 or 2");
         cfg.cellOrder = s->cellOrder;
         auto sol = std::make_unique<afar_solver>();
-        sol->dev = make_device_solver(ra, cfg);
         sol->perm = h->m.perm;
         sol->nc = h->m.nc;
         sol->nb = h->m.nb;
+        sol->dev = make_device_solver(ra, cfg);
         *out = sol.release();
     });
+}
+
+int afar_solver_create(const afar_mesh* h, const afar_schemes* s, const double* W0, afar_solver** out) {
+    return solver_create(h, s, W0, nullptr, out);
+}
+
+int afar_solver_create_on_stream(const afar_mesh* h, const afar_schemes* s, const double* W0, void* stream,
+                                 afar_solver** out) {
+    if (!stream) return guard([] { need(false, "stream is NULL (use afar_solver_create for a stream of its own)"); });
+    return solver_create(h, s, W0, static_cast<hipStream_t>(stream), out);
 }
 
 // ---------------------------------------------------------------- batches
@@ -708,7 +723,10 @@ int afar_batch_profile(afar_batch* b, int32_t reps, double* ms, int32_t* count) 
     });
 }
 
-void afar_batch_destroy(afar_batch* b) { delete b; }
+void afar_batch_destroy(afar_batch* b) {
+    std::lock_guard<std::mutex> lk(runtime_mutex());  // device frees
+    delete b;
+}
 
 int afar_solver_query(afar_solver* h, int32_t* idle) {
     return guard([&] {
@@ -880,22 +898,33 @@ int afar_solver_fetch(afar_solver* h, const char* name, void* out, size_t bytes)
     });
 }
 
-void afar_solver_destroy(afar_solver* h) { delete h; }
+void afar_solver_destroy(afar_solver* h) {
+    std::lock_guard<std::mutex> lk(runtime_mutex());  // device frees
+    delete h;
+}
 
 // ---------------------------------------------------------------- seam 2
 
-int afar_set_device_field_data(afar_device** dev, const afar_field_pointer* host, int32_t cellsNum,
-                               int32_t facesNum, int32_t totalBoundaryFacesNum) {
+// Counts arrive as 64-bit integers; the device layout indexes cells and faces
+// with 32-bit ids (the reference's int), so larger meshes are refused.
+static void need_id_range(int64_t nc, int64_t nf, int64_t nb) {
+    if (nc + nb > INT32_MAX || nf + nb > INT32_MAX)
+        throw Error(AFAR_ERR_LIMIT, "mesh too large for 32-bit cell and face ids");
+}
+
+int afar_set_device_field_data(afar_device** dev, const afar_field_pointer* host, int64_t cellsNum,
+                               int64_t facesNum, int64_t totalBoundaryFacesNum) {
     return guard([&] {
         need(dev && host, "NULL argument");
         need(cellsNum > 0 && facesNum >= 0 && totalBoundaryFacesNum >= 0, "bad sizes");
+        need_id_range(cellsNum, facesNum, totalBoundaryFacesNum);
         need(host->CELL && host->FACE && host->cellVolume && host->FD && host->cFD, "geometry/state arrays missing");
         auto d = std::make_unique<afar_device>();
         RefArrays& ra = d->ra;
         const size_t nc = cellsNum, nf = facesNum, nb = totalBoundaryFacesNum;
-        ra.nc = cellsNum;
-        ra.nf = facesNum;
-        ra.nb = totalBoundaryFacesNum;
+        ra.nc = (int)cellsNum;
+        ra.nf = (int)facesNum;
+        ra.nb = (int)totalBoundaryFacesNum;
         ra.CELL.assign(host->CELL, host->CELL + 2 * (nc + nb));
         ra.FACE.assign(host->FACE, host->FACE + 5 * (nf + nb));
         ra.cellVolume.assign(host->cellVolume, host->cellVolume + nc);
@@ -916,15 +945,24 @@ int afar_device_configure(afar_device* dev, int32_t precision, int32_t tile) {
     });
 }
 
-int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* host, int32_t cellsNum,
-                                 int32_t facesNum, int32_t patchesNum, int32_t totalBoundaryFacesNum,
-                                 int32_t maxStencilSize, int32_t maxCompactStencilSize,
-                                 int32_t maxLocalBlockStencilSize, int32_t maxCompactLocalBlockStencilSize) {
+int afar_device_set_stream(afar_device* dev, void* stream) {
+    return guard([&] {
+        need(dev != nullptr, "NULL device");
+        need(!dev->dev, "set the stream before setDeviceFieldData #2");
+        dev->cfg.stream = static_cast<hipStream_t>(stream);
+    });
+}
+
+int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* host, int64_t cellsNum,
+                                 int64_t facesNum, int64_t patchesNum, int64_t totalBoundaryFacesNum,
+                                 int64_t maxStencilSize, int64_t maxCompactStencilSize,
+                                 int64_t maxLocalBlockStencilSize, int64_t maxCompactLocalBlockStencilSize) {
     (void)maxCompactLocalBlockStencilSize;
     return guard([&] {
         need(dev && host, "NULL argument");
         RefArrays& ra = dev->ra;
         need(cellsNum == ra.nc && facesNum == ra.nf && totalBoundaryFacesNum == ra.nb, "sizes differ from upload #1");
+        need(patchesNum >= 0 && patchesNum <= INT32_MAX, "patchesNum out of range");
         need(maxStencilSize > 0 && maxStencilSize <= kMaxValidStencil, "maxStencilSize out of range");
         need(maxCompactStencilSize > 0 && maxCompactStencilSize <= maxStencilSize, "maxCompactStencilSize out of range");
         need(host->neighbour && host->boundaryFaceNeiLabel && host->boundaryFacesType && host->stencilSize &&
@@ -932,9 +970,9 @@ int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* hos
              "connectivity arrays missing");
         need(host->RBFbasis || host->RBFbasis64, "RBF basis missing");
         const size_t nc = ra.nc, nf = ra.nf, nb = ra.nb;
-        ra.npatch = patchesNum;
-        ra.maxS = maxStencilSize;
-        ra.maxC = maxCompactStencilSize;
+        ra.npatch = (int)patchesNum;
+        ra.maxS = (int)maxStencilSize;
+        ra.maxC = (int)maxCompactStencilSize;
         ra.neighbour.assign(host->neighbour, host->neighbour + 2 * nf);
         if (host->boundaryFacesNum) ra.boundaryFacesNum.assign(host->boundaryFacesNum, host->boundaryFacesNum + patchesNum);
         ra.boundaryFaceNeiLabel.assign(host->boundaryFaceNeiLabel, host->boundaryFaceNeiLabel + nb);
@@ -952,8 +990,9 @@ int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* hos
             ra.stencil.assign(host->globalStencil, host->globalStencil + (size_t)maxStencilSize * nc);
         } else {
             need(host->localStencil && host->extendStencil, "stencil tables missing");
-            need(maxLocalBlockStencilSize >= 128, "maxLocalBlockStencilSize below the block size");
-            global_stencil_from_blocks((int)nc, maxStencilSize, maxLocalBlockStencilSize, ra.stencilSize.data(),
+            need(maxLocalBlockStencilSize >= 128 && maxLocalBlockStencilSize <= INT32_MAX,
+                 "maxLocalBlockStencilSize out of range");
+            global_stencil_from_blocks((int)nc, (int)maxStencilSize, (int)maxLocalBlockStencilSize, ra.stencilSize.data(),
                                        host->localStencil, host->extendStencil, ra.stencil.data());
         }
         for (size_t c = 0; c < nc; ++c)
@@ -989,10 +1028,13 @@ int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* hos
 }
 
 int afar_free_field_data(afar_device* dev) {
-    return guard([&] { delete dev; });
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(runtime_mutex());  // device frees
+        delete dev;
+    });
 }
 
-int afar_adjust_time_step(afar_device* dev, int32_t facesNum, double R, double Cv, double CFL, double* deltaT) {
+int afar_adjust_time_step(afar_device* dev, int64_t facesNum, double R, double Cv, double CFL, double* deltaT) {
     return guard([&] {
         need(dev && dev->dev && deltaT, "device not initialised");
         need(facesNum == dev->ra.nf, "facesNum differs from the upload");
@@ -1001,9 +1043,9 @@ int afar_adjust_time_step(afar_device* dev, int32_t facesNum, double R, double C
     });
 }
 
-int afar_gpu_evolve(afar_device* dev, double R, double Cv, double deltaT, int32_t cellsNum, int32_t facesNum,
-                    int32_t totalBoundaryFacesNum, int32_t maxStencilSize, int32_t maxCompactStencilSize,
-                    int32_t maxLocalBlockStencilSize, int32_t maxCompactLocalBlockStencilSize) {
+int afar_gpu_evolve(afar_device* dev, double R, double Cv, double deltaT, int64_t cellsNum, int64_t facesNum,
+                    int64_t totalBoundaryFacesNum, int64_t maxStencilSize, int64_t maxCompactStencilSize,
+                    int64_t maxLocalBlockStencilSize, int64_t maxCompactLocalBlockStencilSize) {
     (void)maxLocalBlockStencilSize;
     (void)maxCompactLocalBlockStencilSize;
     return guard([&] {
@@ -1017,8 +1059,8 @@ int afar_gpu_evolve(afar_device* dev, double R, double Cv, double deltaT, int32_
     });
 }
 
-int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* dev, int32_t cellsNum,
-                                        int32_t totalBoundaryFacesNum) {
+int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* dev, int64_t cellsNum,
+                                        int64_t totalBoundaryFacesNum) {
     return guard([&] {
         need(dev && dev->dev && host, "device not initialised");
         need(cellsNum == dev->ra.nc && totalBoundaryFacesNum == dev->ra.nb, "sizes differ from the upload");
@@ -1031,6 +1073,7 @@ int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* d
 int afar_selftest_math(const double* x, double* out, int32_t n) {
     return guard([&] {
         need(n >= 0 && x && out, "bad arguments");
+        std::lock_guard<std::mutex> lk(runtime_mutex());  // allocations, a device-wide wait, frees
         double *dx = nullptr, *dout = nullptr;
         check_hip(hipMalloc(&dx, sizeof(double) * (n + 1)), "hipMalloc");
         hipError_t e = hipMalloc(&dout, sizeof(double) * 4 * (n + 1));

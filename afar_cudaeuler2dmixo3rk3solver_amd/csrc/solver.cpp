@@ -18,6 +18,11 @@ void check_hip(hipError_t e, const char* what) {
     if (e != hipSuccess) throw Error(2, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+std::mutex& runtime_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 namespace {
 
 struct DevMem {
@@ -55,9 +60,15 @@ class SolverImpl final : public DeviceSolver {
             bi_.bface_off = {0, ra.nb};
         }
         L_ = make_dev_layout(ra, cfg.tile);
-        check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+        if (cfg.stream) {
+            stream_ = cfg.stream;  // the caller's stream: used, never destroyed
+        } else {
+            check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+            own_stream_ = true;
+        }
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
+        for (hipEvent_t& e : evq_) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         // LDS of the reconstruction: tile + halo states, then the tile's stencil ids
         lds_recon_ = (size_t)(L_.tile + L_.maxHalo) * sizeof(v4) + (size_t)L_.tile * L_.maxS * sizeof(uint16_t);
         lds_m_off_ = (lds_recon_ + 15) & ~(size_t)15;
@@ -102,10 +113,12 @@ class SolverImpl final : public DeviceSolver {
         if (graph_) (void)hipGraphDestroy(graph_);
         if (stream_) {
             (void)hipStreamSynchronize(stream_);
-            (void)hipStreamDestroy(stream_);
+            if (own_stream_) (void)hipStreamDestroy(stream_);
         }
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
+        for (hipEvent_t e : evq_)
+            if (e) (void)hipEventDestroy(e);
     }
 
     double compute_dt() override {
@@ -126,7 +139,7 @@ class SolverImpl final : public DeviceSolver {
 
     void evolve_auto(int nsteps) override {
         if (!graph_exec_) capture();
-        for (int i = 0; i < nsteps; ++i) check_hip(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+        for (int i = 0; i < nsteps; ++i) launch_graph();
     }
 
     void synchronize() override { check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
@@ -265,7 +278,7 @@ class SolverImpl final : public DeviceSolver {
         synchronize();
         check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord");
         if (graph) {
-            for (int i = 0; i < nsteps; ++i) check_hip(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+            for (int i = 0; i < nsteps; ++i) launch_graph();
         } else {
             for (int i = 0; i < nsteps; ++i) enqueue_stages(0.0, P_.dt);
         }
@@ -300,6 +313,7 @@ class SolverImpl final : public DeviceSolver {
         struct Cleanup {
             Copy* c;
             ~Cleanup() {
+                std::lock_guard<std::mutex> lk(runtime_mutex());
                 for (int b = 0; b < 2; ++b) {
                     if (c[b].ge) (void)hipGraphExecDestroy(c[b].ge);
                     if (c[b].g) (void)hipGraphDestroy(c[b].g);
@@ -309,6 +323,7 @@ class SolverImpl final : public DeviceSolver {
             }
         } cleanup{cp};
         synchronize();
+        std::unique_lock<std::mutex> build_lock(runtime_mutex());
         for (int b = 0; b < 2; ++b) {
             for (hipEvent_t& e : cp[b].ev) check_hip(hipEventCreate(&e), "hipEventCreate");
             check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -317,6 +332,7 @@ class SolverImpl final : public DeviceSolver {
             check_hip(hipStreamEndCapture(stream_, &cp[b].g), "hipStreamEndCapture");
             check_hip(hipGraphInstantiate(&cp[b].ge, cp[b].g, nullptr, nullptr, 0), "hipGraphInstantiate");
         }
+        build_lock.unlock();
         auto collect = [&](Copy& c) {
             check_hip(hipEventSynchronize(c.ev[NK]), "hipEventSynchronize");
             for (int j = 0; j < NK; ++j) {
@@ -380,6 +396,7 @@ class SolverImpl final : public DeviceSolver {
         cfg_.Cv = Cv;
         cfg_.CFL = CFL;
         synchronize();
+        std::lock_guard<std::mutex> lk(runtime_mutex());
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
         graph_exec_ = nullptr;
@@ -479,8 +496,23 @@ class SolverImpl final : public DeviceSolver {
         }
     }
 
+    // One graph launch.  At most 2 * kGraphBlock launches are left queued: after
+    // every kGraphBlock launches an event is recorded, and before a third
+    // block starts the host waits for the first to finish (a long evolve_auto
+    // otherwise queues tens of thousands of kernel dispatches at once).
+    static constexpr int kGraphBlock = 256;
+    void launch_graph() {
+        check_hip(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+        if (++launches_ % kGraphBlock) return;
+        hipEvent_t& e = evq_[blocks_ & 1];
+        if (blocks_ >= 2) check_hip(hipEventSynchronize(e), "hipEventSynchronize");
+        check_hip(hipEventRecord(e, stream_), "hipEventRecord");
+        ++blocks_;
+    }
+
     void capture() {
         synchronize();
+        std::lock_guard<std::mutex> lk(runtime_mutex());
         check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         enqueue_stages(0.0, P_.dt);
         check_hip(hipStreamEndCapture(stream_, &graph_), "hipStreamEndCapture");
@@ -655,7 +687,10 @@ class SolverImpl final : public DeviceSolver {
     DevLayout L_;
     DevPtrs<R> P_{};
     hipStream_t stream_ = nullptr;
+    bool own_stream_ = false;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    hipEvent_t evq_[2] = {nullptr, nullptr};  // ends of the last two blocks of graph launches
+    long long launches_ = 0, blocks_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
     size_t lds_recon_ = 0, lds_bvd_ = 0, lds_m_off_ = 0, lds_bvd_sten_off_ = 0;
@@ -811,7 +846,10 @@ std::unique_ptr<DeviceSolver> make_impl(const RefArrays& ra, const SolverConfig&
 std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg,
                                                  const BatchIndex* bi) {
     if (ra.nc <= 0) throw Error(1, "empty mesh");
-    if (cfg.cellOrder == 1) return make_impl(ra, cfg, bi);
+    if (cfg.cellOrder == 1) {
+        std::lock_guard<std::mutex> lk(runtime_mutex());
+        return make_impl(ra, cfg, bi);
+    }
     BatchIndex one;
     if (!bi) {
         one.cell_off = {0, ra.nc};
@@ -825,6 +863,7 @@ std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const Solv
     std::unique_ptr<DeviceSolver> in;
     {
         const RefArrays rl = relabel(ra, r);
+        std::lock_guard<std::mutex> lk(runtime_mutex());
         in = make_impl(rl, cfg, bi);
     }
     return std::unique_ptr<DeviceSolver>(new RelabeledSolver(std::move(in), std::move(r), ra.nc, ra.nf, ra.nb));

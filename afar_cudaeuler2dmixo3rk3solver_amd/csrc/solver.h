@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,7 +22,16 @@ struct SolverConfig {
     int cellOrder = 0;   // device cell order: 0 compact tiles (Hilbert curve) with blocks of 32
                          // (FP64) or 64 (mixed) cells ordered by face count and stencil size,
                          // 1 the mesh's order, 2 compact tiles in Hilbert order throughout
+    hipStream_t stream = nullptr;  // the caller's stream (not owned), or nullptr: the solver creates its own
 };
+
+// Held by every library call that allocates or frees device memory, creates
+// or destroys a solver, or captures or destroys a graph.  HIP frees
+// synchronise the device, and a stream under capture must not be waited on:
+// serialising these calls makes solvers safe to create, step and destroy from
+// different host threads.  Kernel and graph launches, copies and queries on a
+// solver's own stream do not take it.
+std::mutex& runtime_mutex();
 
 // kernel kinds reported by profile()
 // Kernel kinds of profile(): the standalone CFL pass (host-requested time

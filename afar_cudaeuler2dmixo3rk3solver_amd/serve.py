@@ -1,7 +1,8 @@
 """Served solves with continuous batching on one GPU.
 
-Requests (a corner flow run to a density-residual tolerance, or a
-four-quadrant Riemann problem run to a fixed time) arrive over time.  Each is
+Requests (a corner flow run until its density residual has fallen by a
+given factor, or a four-quadrant Riemann problem run to a fixed time) arrive
+over time.  Each is
 prepared on the host (mesh, RCM, stencils, RBF precompute), then admitted to
 the GPU when its device footprint fits in the HBM budget and a slot is free.
 Admitted requests run concurrently, each on its own HIP stream with its state
@@ -15,9 +16,11 @@ checked is fixed per request.
     python -m afar_cudaeuler2dmixo3rk3solver_amd.serve [--rate 20] [--duration 120] [--nc-min 1e4] [--nc-max 2e6]
 
 prints one JSON line per request and a summary line with p50/p99
-time-to-solution (arrival to result, precompute included), the aggregate
-cell-steps/s and the L1 density error of a sample of requests against the
-all-FP64 solve.
+time-to-solution (arrival to result, precompute included), how many requests
+met their stop criterion (converged) or failed, the aggregate cell-steps/s
+and the L1 density error of a sample of requests against the all-FP64 solve.
+A request whose preparation or solve raises is recorded as failed (its error
+message kept) and the loop goes on.
 """
 import argparse
 import faulthandler
@@ -43,14 +46,17 @@ class Request:
     nc_target: int
     seed: int
     t_end: float = 0.0        # riemann
-    res_tol: float = 0.0      # corner
-    max_steps: int = 0        # corner
+    res_tol: float = 0.0      # corner: stop when residual / first residual < res_tol
+    max_steps: int = 0        # corner: step cap
     # filled while serving
     times: dict = field(default_factory=dict)
     steps: int = 0
     cells: int = 0
     result: object = None
-    residual: float = float("nan")
+    residual: float = float("nan")   # last density residual (corner)
+    residual0: float = float("nan")  # density residual after the first chunk (corner)
+    converged: bool = False          # the stop criterion was met (not the step cap)
+    error: str = ""                  # set when preparation or solving raised
 
 
 def request_stream(rate=20.0, duration=120.0, seed=3, nc_min=1e4, nc_max=2e6, t_end=0.02, res_tol=1e-8,
@@ -144,22 +150,38 @@ class Server:
         req.steps += k
 
     def _advance(self, req, g):
-        """Called when the solver's stream is idle: True when the request is done."""
+        """Called when the solver's stream is idle: True when the request is done.
+        Corner requests stop when the density residual has fallen below res_tol
+        times its value after the first chunk (the residual's absolute size
+        depends on the mesh and the inflow), or at the step cap."""
         if req.kind == "riemann":
             if g.device_time() >= req.t_end:
                 req.steps = g.device_steps()  # steps that moved the clock
+                req.converged = True
                 return True
         else:
             req.residual = g.residual_norm()
-            if req.residual < req.res_tol or req.steps >= req.max_steps:
+            if math.isnan(req.residual0):
+                req.residual0 = req.residual
+            if req.residual0 > 0 and req.residual < req.res_tol * req.residual0:
+                req.converged = True
+                return True
+            if req.steps >= req.max_steps or not math.isfinite(req.residual):
                 return True
         self._enqueue(req, g)
         return False
 
     def _finish(self, req, g, t0):
-        req.result = g.conserved()
-        g.close()
-        req.times["finish"] = time.perf_counter() - t0
+        try:
+            req.result = g.conserved()
+        finally:
+            g.close()
+            req.times["finish"] = time.perf_counter() - t0
+
+    @staticmethod
+    def _fail(req, exc, t0):
+        req.error = "%s: %s" % (type(exc).__name__, exc)
+        req.times.setdefault("finish", time.perf_counter() - t0)
 
     def solve(self, req, mesh, W0, t0):
         """One request alone, with the same chunking as in the loop."""
@@ -176,6 +198,7 @@ class Server:
         t0 = time.perf_counter()
         prep_q = queue.Queue()
         ready_q = queue.Queue()
+        failed_q = queue.Queue()
 
         def arrivals():
             for r in requests:
@@ -188,17 +211,26 @@ class Server:
                 prep_q.put(None)
 
         def preparer():
-            while True:
-                r = prep_q.get()
-                if r is None:
-                    ready_q.put(None)
-                    return
-                r.times["prep_start"] = time.perf_counter() - t0
-                # with the GPU precompute only the mesh is built here (no GPU
-                # calls on this thread); the scheduler precomputes on admission
-                mesh, W0 = build_case(r, None if self.gpu_precompute else self.schemes_of(r))
-                r.times["prepared"] = time.perf_counter() - t0
-                ready_q.put([r, mesh, W0, None if self.gpu_precompute else footprint_bytes(mesh, self.precision)])
+            try:
+                while True:
+                    r = prep_q.get()
+                    if r is None:
+                        return
+                    r.times["prep_start"] = time.perf_counter() - t0
+                    try:
+                        # with the GPU precompute only the mesh is built here (no GPU
+                        # calls on this thread); the scheduler precomputes on admission
+                        mesh, W0 = build_case(r, None if self.gpu_precompute else self.schemes_of(r))
+                        need = None if self.gpu_precompute else footprint_bytes(mesh, self.precision)
+                    except Exception as e:  # a request that cannot be prepared fails alone
+                        r.times["prepared"] = time.perf_counter() - t0
+                        self._fail(r, e, t0)
+                        failed_q.put(r)
+                        continue
+                    r.times["prepared"] = time.perf_counter() - t0
+                    ready_q.put([r, mesh, W0, need])
+            finally:
+                ready_q.put(None)  # always posted, so the scheduler's producer count reaches zero
 
         feeder = threading.Thread(target=arrivals, daemon=True)
         preps = [threading.Thread(target=preparer, daemon=True) for _ in range(self.prep_workers)]
@@ -213,6 +245,11 @@ class Server:
         producers = self.prep_workers
         while producers > 0 or waiting or active:
             progressed = False
+            while True:  # requests whose preparation failed
+                try:
+                    done.append(failed_q.get_nowait())
+                except queue.Empty:
+                    break
             while True:  # collect prepared requests
                 try:
                     item = ready_q.get_nowait()
@@ -224,22 +261,42 @@ class Server:
                     waiting.append(item)
             while waiting and len(active) < self.max_active:
                 if waiting[0][3] is None:  # precompute on the GPU, then size it
-                    waiting[0][1].precompute(self.schemes_of(waiting[0][0]), gpu=True)
-                    waiting[0][3] = footprint_bytes(waiting[0][1], self.precision)
+                    try:
+                        waiting[0][1].precompute(self.schemes_of(waiting[0][0]), gpu=True)
+                        waiting[0][3] = footprint_bytes(waiting[0][1], self.precision)
+                    except Exception as e:
+                        r = waiting.pop(0)[0]
+                        r.times["admitted"] = time.perf_counter() - t0
+                        self._fail(r, e, t0)
+                        done.append(r)
+                        progressed = True
+                        continue
                     waiting[0][0].times["precomputed"] = time.perf_counter() - t0
                 if used + waiting[0][3] > self.budget and active:
                     break
                 r, mesh, W0, need = waiting.pop(0)
                 r.times["admitted"] = time.perf_counter() - t0
-                g = self._start(r, mesh, W0, t0)
+                progressed = True
+                try:
+                    g = self._start(r, mesh, W0, t0)
+                except Exception as e:  # e.g. the device is out of memory for this request
+                    self._fail(r, e, t0)
+                    done.append(r)
+                    continue
                 active.append((r, g, need))
                 used += need
                 self.peak_active = max(self.peak_active, len(active))
-                progressed = True
             still = []
             for r, g, need in active:
-                if g.idle() and self._advance(r, g):
-                    self._finish(r, g, t0)
+                try:
+                    finished = g.idle() and self._advance(r, g)
+                    if finished:
+                        self._finish(r, g, t0)
+                except Exception as e:
+                    g.close()
+                    self._fail(r, e, t0)
+                    finished = True
+                if finished:
                     used -= need
                     done.append(r)
                     self.log(json.dumps(summary_row(r)))
@@ -262,16 +319,27 @@ class Server:
         feeder.join()
         for p in preps:
             p.join()
+        while not failed_q.empty():
+            done.append(failed_q.get_nowait())
         return sorted(done, key=lambda r: r.rid), time.perf_counter() - t0
 
 
 def summary_row(r):
     tts = r.times["finish"] - r.arrival
-    return dict(rid=r.rid, kind=r.kind, cells=r.cells, steps=r.steps, arrival=round(r.arrival, 3),
-                tts_s=round(tts, 3), prep_s=round(r.times["prepared"] - r.times["prep_start"], 3),
-                queue_s=round(r.times["admitted"] - r.times["prepared"], 3),
-                gpu_s=round(r.times["finish"] - r.times["start"], 3),
-                residual=None if math.isnan(r.residual) else r.residual)
+    out = dict(rid=r.rid, kind=r.kind, cells=r.cells, steps=r.steps, arrival=round(r.arrival, 3),
+               tts_s=round(tts, 3), converged=r.converged)
+    if "prepared" in r.times and "prep_start" in r.times:
+        out["prep_s"] = round(r.times["prepared"] - r.times["prep_start"], 3)
+    if "admitted" in r.times and "prepared" in r.times:
+        out["queue_s"] = round(r.times["admitted"] - r.times["prepared"], 3)
+    if "start" in r.times:
+        out["gpu_s"] = round(r.times["finish"] - r.times["start"], 3)
+    if not math.isnan(r.residual):
+        out["residual"] = r.residual
+        out["residual_rel"] = r.residual / r.residual0 if r.residual0 > 0 else None
+    if r.error:
+        out["error"] = r.error
+    return out
 
 
 def fp64_reference(req, chunk=50):
@@ -292,7 +360,8 @@ def main(argv=None):
     ap.add_argument("--nc-min", type=float, default=1e4)
     ap.add_argument("--nc-max", type=float, default=2e6)
     ap.add_argument("--t-end", type=float, default=0.02, help="end time of the Riemann requests")
-    ap.add_argument("--res-tol", type=float, default=1e-8)
+    ap.add_argument("--res-tol", type=float, default=1e-8,
+                    help="corner requests stop when the density residual falls below this times its first value")
     ap.add_argument("--max-steps", type=int, default=400, help="step cap of the corner requests")
     ap.add_argument("--max-active", type=int, default=8)
     ap.add_argument("--chunk", type=int, default=50)
@@ -306,22 +375,31 @@ def main(argv=None):
     srv = Server(max_active=args.max_active, chunk=args.chunk, log=lambda m: print(m, flush=True),
                  gpu_precompute=args.gpu_precompute)
     done, wall = srv.run(reqs)
-    tts = np.array([r.times["finish"] - r.arrival for r in done])
-    cell_steps = float(sum(r.cells * r.steps for r in done))
-    # a seeded sample of ceil(fraction x requests) requests, at least one
+    ok = [r for r in done if not r.error]
+    tts = np.array([r.times["finish"] - r.arrival for r in ok])
+    cell_steps = float(sum(r.cells * r.steps for r in ok))
+    # a seeded sample of ceil(fraction x requests) solved requests, at least one
     rng = np.random.default_rng(args.seed + 1)
-    k = min(len(done), max(1, int(np.ceil(args.sample * len(done)))))
-    order = sorted(done, key=lambda r: r.rid)
-    sample = [order[i] for i in sorted(rng.choice(len(order), size=k, replace=False))] if done else []
+    k = min(len(ok), max(1, int(np.ceil(args.sample * len(ok)))))
+    order = sorted(ok, key=lambda r: r.rid)
+    sample = [order[i] for i in sorted(rng.choice(len(order), size=k, replace=False))] if ok else []
     errs = []
     for r in sample:
         ref = fp64_reference(r, args.chunk)
         d = np.abs(r.result[:, 0] - ref.result[:, 0]).sum() / np.abs(ref.result[:, 0]).sum()
         errs.append(float(d))
     print(json.dumps(dict(
-        device=device_info()[0].strip(), requests=len(done), rate=args.rate, duration=args.duration,
-        nc_range=[args.nc_min, args.nc_max], wall_s=round(wall, 2), p50_tts_s=round(float(np.percentile(tts, 50)), 3),
-        p99_tts_s=round(float(np.percentile(tts, 99)), 3), cell_steps_per_s=round(cell_steps / wall, 1),
+        device=device_info()[0].strip(), requests=len(done), failed=len(done) - len(ok),
+        converged=dict(corner=sum(r.converged for r in ok if r.kind == "corner"),
+                       corner_of=sum(r.kind == "corner" for r in ok),
+                       riemann=sum(r.converged for r in ok if r.kind == "riemann"),
+                       riemann_of=sum(r.kind == "riemann" for r in ok)),
+        rate=args.rate, duration=args.duration, res_tol_rel=args.res_tol, corner_step_cap=args.max_steps,
+        riemann_t_end=args.t_end,
+        nc_range=[args.nc_min, args.nc_max], wall_s=round(wall, 2),
+        p50_tts_s=round(float(np.percentile(tts, 50)), 3) if len(tts) else None,
+        p99_tts_s=round(float(np.percentile(tts, 99)), 3) if len(tts) else None,
+        cell_steps_per_s=round(cell_steps / wall, 1),
         Mcell_steps_per_s=round(cell_steps / wall / 1e6, 2), peak_active=srv.peak_active,
         fp64_sample=len(errs), rho_L1_rel_vs_fp64_max=max(errs) if errs else None,
         rho_L1_rel_vs_fp64_median=float(np.median(errs)) if errs else None)), flush=True)

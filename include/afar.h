@@ -17,8 +17,22 @@
  *
  * Every function returns AFAR_OK (0) or an error code and never exits the
  * process; afar_last_error() gives the message of the calling thread's last
- * failure.  Handles are independent: calls on different handles may run on
- * different host threads.  All sizes are the reference's 32-bit counts.
+ * failure.
+ *
+ * Threading.  Handles are independent: different handles may be created,
+ * stepped and destroyed on different host threads at the same time.  The
+ * calls that allocate or free device memory, create or destroy a solver,
+ * batch or device, or capture a step graph take one process-wide lock, so
+ * they never overlap each other (a HIP free waits for the whole device, and
+ * a stream being captured must not be waited on); kernel launches, copies
+ * and queries on a handle's own stream do not take it.  One handle must not
+ * be used by two threads at once.  Each solver runs on one HIP stream: its
+ * own (afar_solver_create), or the caller's (afar_solver_create_on_stream,
+ * afar_device_set_stream), which the library uses but never destroys.
+ *
+ * Sizes.  Cell and face counts of seam 2 are 64-bit; the device layout
+ * indexes cells and faces with 32-bit ids (the reference's int), so meshes
+ * with more than 2^31 - 1 cells + boundary faces are refused (AFAR_ERR_LIMIT).
  */
 #ifndef AFAR_H
 #define AFAR_H
@@ -188,13 +202,23 @@ int afar_pinv(int32_t n, const double* A, double* G);
  * per cell in the caller's order (the 0/ fields); ghost states come from the
  * boundary table.  The mesh must have been precomputed with the same schemes. */
 int afar_solver_create(const afar_mesh* m, const afar_schemes* s, const double* W0, afar_solver** out);
+/* The same solver on the caller's stream (a hipStream_t passed as void*; not
+ * NULL): every kernel, graph launch and copy of the solver is ordered on it,
+ * so the caller can order the solver's work with its own.  The library never
+ * destroys the stream; it must outlive the solver.  The reference has no
+ * counterpart (its kernels run on the legacy default stream,
+ * GPUevolve.cu:193,299); for callers that schedule several solvers, as
+ * batching and serving do. */
+int afar_solver_create_on_stream(const afar_mesh* m, const afar_schemes* s, const double* W0, void* stream,
+                                 afar_solver** out);
 /* basicCUDAsolver::setDeltaT (basicCUDAsolver.cpp:854-859): global CFL time step. */
 int afar_solver_set_delta_t(afar_solver* h, double* dt);
 /* basicCUDAsolver::evolve (basicCUDAsolver.cpp:862-871): one SSP-RK3 step of size dt.
  * Asynchronous: returns once the work is queued on the solver's stream. */
 int afar_solver_evolve(afar_solver* h, double dt);
 /* nsteps of (CFL time step + RK3 step) with the time step kept on the
- * device; one captured hipGraph per step.  Asynchronous. */
+ * device; one captured hipGraph per step.  Asynchronous, with at most 512
+ * steps left queued: a longer call waits for earlier steps as it goes. */
 int afar_solver_evolve_auto(afar_solver* h, int32_t nsteps);
 /* Time step used by the last evolve_auto step (synchronises). */
 int afar_solver_last_dt(afar_solver* h, double* dt);
@@ -317,28 +341,35 @@ struct afar_field_pointer {
     double* boundaryValue;             /* 4*nb fixed values of label 6 */
 };
 
-/* setDeviceFieldData #1 (GPUevolve.cu:13-64): geometry and state. */
-int afar_set_device_field_data(afar_device** dev, const afar_field_pointer* host, int32_t cellsNum,
-                               int32_t facesNum, int32_t totalBoundaryFacesNum);
+/* setDeviceFieldData #1 (GPUevolve.cu:13-64): geometry and state.  The counts
+ * are the reference's int arguments widened to 64 bits (a C/C++ caller
+ * passing int compiles unchanged). */
+int afar_set_device_field_data(afar_device** dev, const afar_field_pointer* host, int64_t cellsNum,
+                               int64_t facesNum, int64_t totalBoundaryFacesNum);
 /* Optional, between the two uploads: precision (0 FP64, 1 FP32-mixed), tile, R, Cv. */
 int afar_device_configure(afar_device* dev, int32_t precision, int32_t tile);
+/* Optional, between the two uploads: run on the caller's stream (a hipStream_t
+ * passed as void*) instead of one of the device's own; NULL keeps the
+ * default.  No reference counterpart (the reference uses the legacy default
+ * stream); the stream must outlive the device. */
+int afar_device_set_stream(afar_device* dev, void* stream);
 /* setDeviceFieldData #2 (GPUevolve.cu:67-148): connectivity, stencils,
  * coefficients; computes the least-squares weights on the device (K3). */
-int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* host, int32_t cellsNum,
-                                 int32_t facesNum, int32_t patchesNum, int32_t totalBoundaryFacesNum,
-                                 int32_t maxStencilSize, int32_t maxCompactStencilSize,
-                                 int32_t maxLocalBlockStencilSize, int32_t maxCompactLocalBlockStencilSize);
+int afar_set_device_stencil_data(afar_device* dev, const afar_field_pointer* host, int64_t cellsNum,
+                                 int64_t facesNum, int64_t patchesNum, int64_t totalBoundaryFacesNum,
+                                 int64_t maxStencilSize, int64_t maxCompactStencilSize,
+                                 int64_t maxLocalBlockStencilSize, int64_t maxCompactLocalBlockStencilSize);
 /* freeFieldData (GPUevolve.cu:151-174): releases the device and the handle. */
 int afar_free_field_data(afar_device* dev);
 /* adjustTimeStep (GPUevolve.cu:177-204). */
-int afar_adjust_time_step(afar_device* dev, int32_t facesNum, double R, double Cv, double CFL, double* deltaT);
+int afar_adjust_time_step(afar_device* dev, int64_t facesNum, double R, double Cv, double CFL, double* deltaT);
 /* GPUevolve (GPUevolve.cu:207-306): one SSP-RK3 step; synchronous like the reference. */
-int afar_gpu_evolve(afar_device* dev, double R, double Cv, double deltaT, int32_t cellsNum, int32_t facesNum,
-                    int32_t totalBoundaryFacesNum, int32_t maxStencilSize, int32_t maxCompactStencilSize,
-                    int32_t maxLocalBlockStencilSize, int32_t maxCompactLocalBlockStencilSize);
+int afar_gpu_evolve(afar_device* dev, double R, double Cv, double deltaT, int64_t cellsNum, int64_t facesNum,
+                    int64_t totalBoundaryFacesNum, int64_t maxStencilSize, int64_t maxCompactStencilSize,
+                    int64_t maxLocalBlockStencilSize, int64_t maxCompactLocalBlockStencilSize);
 /* copyFieldDataDeviceToHost (GPUevolve.cu:309-318): FD and shockIndicator. */
-int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* dev, int32_t cellsNum,
-                                        int32_t totalBoundaryFacesNum);
+int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* dev, int64_t cellsNum,
+                                        int64_t totalBoundaryFacesNum);
 
 /* ------------------------------------------------------------------ diagnostics */
 /* Evaluates exp, erfc, |x|^(1/2) and 1/x of the FP64 flux path on the device: out 4*n. */

@@ -90,3 +90,28 @@ def test_stencil_over_twenty_is_an_error(afar):
     # a valid mesh still builds afterwards
     ok = afar.Mesh.riemann(n=6, seed=2)
     ok.precompute(afar.schemes(cOrder=7, fOrder=2))
+
+
+def test_stream_and_64bit_count_checks(afar):
+    """The caller-stream entry points and the 64-bit seam-2 counts reject bad
+    input before any device call."""
+    import afar_cudaeuler2dmixo3rk3solver_amd._lib as L_
+    L = afar.lib()
+    m = afar.Mesh.riemann(n=4)
+    sc = afar.schemes(cOrder=7, fOrder=2)
+    m.precompute(sc)
+    W0 = np.tile([1.0, 0.0, 0.0, 1.0], (m.nc, 1))
+    h = C.c_void_p()
+    rc = L.afar_solver_create_on_stream(m.handle, C.byref(sc), W0.ctypes.data_as(C.POINTER(C.c_double)), None,
+                                        C.byref(h))
+    assert rc == L_.AFAR_OK + 1 and b"stream is NULL" in L.afar_last_error()
+    assert L.afar_device_set_stream(None, None) == 1
+    # more cells than 32-bit ids can address: refused with AFAR_ERR_LIMIT, host arrays not read
+    fp = L_.FieldPointer()
+    dummy = np.zeros(16)
+    for name in ("CELL", "FACE", "cellVolume", "FD", "cFD"):
+        setattr(fp, name, dummy.ctypes.data_as(C.POINTER(C.c_double)))
+    dev = C.c_void_p()
+    rc = L.afar_set_device_field_data(C.byref(dev), C.byref(fp), 2 ** 31, 10, 4)
+    assert rc == 3 and b"32-bit" in L.afar_last_error()
+    assert not dev.value

@@ -52,3 +52,29 @@ def test_footprint_grows_with_mesh(afar):
         # at least the state, coefficients and face arrays: > 400 B per cell in FP64
         assert f64 > 400 * m.nc
         prev = f32
+
+
+def test_failed_preparation_does_not_stall_the_loop(afar, monkeypatch):
+    """A request whose preparation raises (here: every one, so no GPU call is
+    made) is recorded as failed with its error, and the request loop ends."""
+    import threading
+
+    from afar_cudaeuler2dmixo3rk3solver_amd import serve
+
+    def broken(req, sc=None):
+        raise RuntimeError("stencil of request %d exceeds the limit" % req.rid)
+
+    monkeypatch.setattr(serve, "build_case", broken)
+    reqs = [serve.Request(k, 0.0, "corner" if k % 2 else "riemann", 1000, k, t_end=0.01, res_tol=1e-8,
+                          max_steps=10) for k in range(5)]
+    srv = serve.Server(precision="fp64", prep_workers=2)
+    out = {}
+    th = threading.Thread(target=lambda: out.setdefault("r", srv.run(reqs, realtime=False)), daemon=True)
+    th.start()
+    th.join(timeout=30)
+    assert not th.is_alive(), "the request loop did not end"
+    done, _ = out["r"]
+    assert [r.rid for r in done] == list(range(5))
+    assert all("exceeds the limit" in r.error and not r.converged for r in done)
+    row = serve.summary_row(done[0])
+    assert "error" in row and "gpu_s" not in row

@@ -1,0 +1,99 @@
+"""GPU: every boundary label and the reference's own example case.
+
+- The cornerFlow case (simulation/cornerFlow/cornerFlow_cuda) at 2 x (40 x
+  20) quads, written by cases.write_corner_flow_case: the C++ application
+  runs the case directory (polyMesh, 0/ fields, controlDict, fvSchemes with
+  cOrder 7, fOrder 5 and patch labels 5 / 2 / 4) to the case's end time
+  t = 0.15, and every time directory it writes equals the oracle's state at
+  that time bit for bit.  maxCo is 0.15: with the case's 0.3 the algorithm
+  loses positivity in its first step (tests/test_cpu_corner_flow_case.py).
+- Labels 0 (isothermal wall), 1 (adiabatic wall) and 3 (the fixed state
+  (1, 3, 0, 1)) on a jittered triangle/quad mesh: 100 FP64 steps, time steps
+  and state bitwise equal to the oracle.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                      "afar_cudaeuler2dmixo3rk3solver_amd", "cudaEuler2DmixO3RK3solver")
+
+
+def read_list(path, vector=False):
+    body = open(path).read().split("List<vector>" if vector else "List<scalar>")[1].split("(", 1)[1]
+    body = body.rsplit(")", 1)[0]
+    if vector:
+        rows = [r.strip().strip("()").split() for r in body.strip().splitlines() if r.strip()]
+        return np.array([[float(a), float(b)] for a, b, _ in rows])
+    return np.array([float(t) for t in body.split()])
+
+
+def test_corner_flow_case_through_application(afar, tmp_path):
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    print("case: cornerFlow 2x(40x20), fOrder 5, labels 5/2/4, application to t=0.15 vs oracle", flush=True)
+    assert os.path.exists(DRIVER), "the C++ application is not built"
+    d = str(tmp_path)
+    cases.write_corner_flow_case(d, nx=40, ny=20, max_co=0.15)
+    r = subprocess.run([DRIVER, "-case", d], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-300:], r.stderr[-300:], flush=True)
+    assert r.returncode == 0
+    # the oracle on the same case, with the application's time loop
+    m = afar.Mesh.read_foam(d)
+    W0, _, _ = m.read_foam_fields(d, "0")
+    for p, lab in enumerate([5, 2, 4]):
+        m.set_patch(p, lab)
+    sc = afar.schemes(cOrder=7, fOrder=5, maxCo=0.15)
+    m.precompute(sc)
+    orc = O.OracleState(m.reference_arrays(sc, W0), sc.R, sc.Cv)
+    written = 0
+    for t, dt, write in cases.foam_time_loop(0.15, 0.01, lambda t: orc.adjust_time_step(0.15)):
+        orc.evolve(dt)
+        if not write:
+            continue
+        tdir = os.path.join(d, "%.6g" % t)
+        assert os.path.isdir(tdir), tdir
+        W = orc.W  # mesh order = the file order of the written case
+        assert np.array_equal(read_list(os.path.join(tdir, "p")), W[:, 0]), tdir
+        assert np.array_equal(read_list(os.path.join(tdir, "T")), W[:, 3]), tdir
+        assert np.array_equal(read_list(os.path.join(tdir, "U"), vector=True), W[:, 1:3]), tdir
+        assert np.array_equal(read_list(os.path.join(tdir, "shockIndicator")), orc.final_types.astype(float)), tdir
+        written += 1
+    assert written == 15
+    assert (orc.final_types < 0).any()  # the KFVS branch ran
+
+
+def test_wall_and_inflow_labels(afar):
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    from afar_cudaeuler2dmixo3rk3solver_amd.solver import BasicCUDASolver
+    print("case: riemann-type 24x24 mesh, labels 3 / 4 / 0 / 1, FP64, 100 steps vs oracle", flush=True)
+    m = afar.Mesh.riemann(n=24, jitter=0.2, seed=4, split_p=0.5)
+    for p, lab in enumerate([3, 4, 0, 1]):  # left inflow, right outlet, bottom and top walls
+        m.set_patch(p, lab)
+    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.4)
+    m.precompute(sc)
+    xy = cases._centroids(m)
+    p = 1.0 + 0.5 * np.exp(-((xy[:, 0] - 0.5) ** 2 + (xy[:, 1] - 0.5) ** 2) / 0.02)
+    W0 = np.stack([p, np.full_like(p, 0.5), np.zeros_like(p), p], axis=1)
+    ref = m.reference_arrays(sc, W0)
+    assert sorted(np.unique(ref["boundaryFacesType"]).tolist()) == [0, 1, 3, 4]
+    orc = O.OracleState(ref, sc.R, sc.Cv)
+    gpu = BasicCUDASolver(m, sc, W0)
+    for s in range(100):
+        dt_o = orc.adjust_time_step(sc.maxCo)
+        dt_g = gpu.set_delta_t()
+        assert dt_o == dt_g, s
+        orc.evolve(dt_o)
+        gpu.evolve(dt_g)
+    gpu.synchronize()
+    nc = m.nc
+    U = gpu.fetch("U", np.float64, (nc, 4))
+    W = gpu.fetch("W", np.float64, (nc + gpu.nb, 4))
+    assert np.array_equal(U, orc.U)
+    assert np.array_equal(W, orc.a["FD"])  # ghosts of every label included
+    assert np.array_equal(gpu.fetch("smooth", np.int8, nc + gpu.nb)[:nc], orc.smooth_flags)

@@ -68,7 +68,9 @@ def test_mixed_precision_corner(afar):
     linf = np.abs(U[:, 0] - orc.U[:, 0]).max() / np.abs(orc.U[:, 0]).max()
     say("  mixed vs FP64 oracle: density L1 rel %.3e, Linf rel %.3e" % (rel, linf))
     assert np.all(np.isfinite(U))
-    assert rel < 1e-3
+    # measured 9.5e-6 (the FP32 arithmetic is deterministic): above the 1e-6 target on this
+    # 20,000-cell mesh, where float rounding enters through the cells near the expansion fan
+    assert rel < 2e-5
 
 
 def test_mixed_stage_fluxes_near_oracle(afar):
