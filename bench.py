@@ -200,13 +200,26 @@ def bandwidth_probe(mib=1024, reps=20):
                       "fill_write_GBps": round(nb / ms["fill"] / 1e6, 1), "ms": {k: round(v, 4) for k, v in ms.items()}}))
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(mesh, sc, W0, budget_s=15.0, max_steps=100):
-    """The FP64 oracle (OpenMP) on the same mesh: steps until budget_s."""
+    """The FP64 oracle (OpenMP) on the same mesh from the same state: steps
+    until budget_s.  Returns the baseline entry and the oracle (its initial
+    conserved state and the state after the timed steps, for the accuracy
+    check)."""
     from oracle import pyoracle as O
     ref = mesh.reference_arrays(sc, W0)
     orc = O.OracleState(ref, sc.R, sc.Cv, boundary_first_order=bool(sc.boundaryFirstOrder))
     del ref
-    orc.run(1, sc.maxCo)  # warm-up step
+    U0 = orc.U.copy()  # mesh order
     t0 = time.perf_counter()
     n = 0
     while n < max_steps:
@@ -216,9 +229,29 @@ def cpu_baseline(mesh, sc, W0, budget_s=15.0, max_steps=100):
             break
     el = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return dict(value=mesh.nc * n / el / 1e6, unit="Mcell-RK3-steps/s", cores=cores, kind="port",
-                sample="%d RK3 steps (with CFL) of the same %d-cell mesh from the same state, FP64 oracle, "
-                       "%d OpenMP threads, %.1f s" % (n, mesh.nc, cores, el))
+    entry = dict(value=mesh.nc * n / el / 1e6, unit="Mcell-RK3-steps/s", cores=cores, kind="port",
+                 cpu=cpu_model(),
+                 sample="%d RK3 steps (with CFL) of the same %d-cell mesh from the same state, FP64 oracle, "
+                        "%d OpenMP threads, %.1f s" % (n, mesh.nc, cores, el))
+    return entry, dict(U0=U0, U=orc.U.copy(), steps=n)
+
+
+def accuracy_vs_oracle(gpu, mesh, check):
+    """The GPU path from the oracle's starting state, as many device-time-step
+    steps as the oracle ran: relative L1 of each conserved variable (density
+    first) against the oracle's state, and whether they are equal bit for bit."""
+    perm = mesh.permutation()
+    U0 = np.empty_like(check["U0"])
+    U0[perm] = check["U0"]
+    gpu.set_conserved(U0)
+    gpu.set_time(0.0)
+    gpu.evolve_auto(check["steps"])
+    gpu.synchronize()
+    U = gpu.conserved()[perm]
+    Uo = check["U"]
+    rel = np.abs(U - Uo).sum(axis=0) / np.abs(Uo).sum(axis=0)
+    return dict(reference="FP64 oracle (CPU), same mesh and starting state", steps=check["steps"],
+                rho_L1_rel=float(rel[0]), L1_rel=[float(v) for v in rel], bitwise_equal=bool(np.array_equal(U, Uo)))
 
 
 def main():
@@ -381,29 +414,44 @@ def main():
     default_layout = (not args.face_quadrature and args.cell_order == 0 and args.tile == 64
                       and args.recon_lanes in (0, 4))
     d = kernels[dom]
+    # The roofline unit is one RK3 step (one graph launch: every kernel of the
+    # step, CFL reduction included).  Its algorithmic bytes are the whole
+    # pipeline's compulsory traffic with no intermediate arrays (static data
+    # streamed once per stage, state read and written), evaluated on this
+    # mesh's stencil and compact sizes, times the cells.
     balg = survey_balg_per_cell_step(st, w)
-    whole_step = {"balg_bytes_per_cell_step": round(balg, 1),
-                  "balg_GBps": round(balg * value * 1e6 / 1e9, 1),
-                  "kernel_alg_bytes_per_cell_step": round(
-                      sum(kb[k] * kernels[k]["count_per_step"] for k in kb) / mesh.nc, 1)}
+    step_s = seconds / args.steps
+    achieved = balg * mesh.nc / step_s / 1e9
+    step_bytes = None
     if default_layout:
-        # measured HBM bytes of one RK3 step (each kernel's committed counter
-        # summary x its launches per step) over the timed step time
+        # measured HBM bytes of one RK3 step: each kernel's committed counter
+        # summary x its launches per step
         tr = {k: measured_traffic(k, args.case, args.precision) for k in kernels if kernels[k]["count_per_step"]}
         if all(v is not None for v in tr.values()):
-            step_bytes = sum(tr[k] * kernels[k]["count_per_step"] for k in tr)
-            gbps = step_bytes / (seconds / args.steps) / 1e9
-            whole_step.update(measured_bytes_per_step=int(step_bytes), measured_GBps=round(gbps, 1),
-                              measured_frac=round(gbps / PEAK_HBM_GBPS, 4))
-    roofline = dict(bound="hbm", achieved=round(d["GBps"], 1), peak=PEAK_HBM_GBPS, unit="GB/s",
-                    frac=round(d["GBps"] / PEAK_HBM_GBPS, 4),
-                    traffic=measured_traffic(dom, args.case, args.precision) if default_layout else None, kernel=dom,
-                    alg_bytes_per_launch=int(d["alg_bytes"]), avg_launch_ms=round(d["avg_ms"], 5),
-                    timing="hipEvents after every kernel of %d graph-replayed steps following the timed ones "
-                           "(%.4f ms per step with the events)" % (args.steps, live_ms / max(args.steps, 1)))
-    cpu = None
+            step_bytes = int(sum(tr[k] * kernels[k]["count_per_step"] for k in tr))
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBPS, unit="GB/s",
+                    frac=round(achieved / PEAK_HBM_GBPS, 4), traffic=step_bytes,
+                    unit_of_work="one RK3 step of %d cells (graph launch, CFL reduction included)" % mesh.nc,
+                    alg_bytes_per_cell_step=round(balg, 1), alg_bytes_per_step=int(balg * mesh.nc),
+                    avg_step_ms=round(step_s * 1e3, 5),
+                    traffic_note="HBM bytes per step from the rocprofv3 counter summary "
+                                 "profiles/riemann1000_%s_hbm_traffic.txt (per kernel x launches per step)"
+                                 % args.precision if step_bytes else None,
+                    traffic_GBps=round(step_bytes / step_s / 1e9, 1) if step_bytes else None,
+                    dominant_kernel=dict(kernel=dom, alg_bytes_per_launch=int(d["alg_bytes"]),
+                                         avg_launch_ms=round(d["avg_ms"], 5), GBps=round(d["GBps"], 1),
+                                         frac=round(d["GBps"] / PEAK_HBM_GBPS, 4),
+                                         traffic=measured_traffic(dom, args.case, args.precision)
+                                         if default_layout else None,
+                                         timing="hipEvents after every kernel of %d graph-replayed steps following "
+                                                "the timed ones (%.4f ms per step with the events)"
+                                                % (args.steps, live_ms / max(args.steps, 1))))
+    whole_step = {"kernel_alg_bytes_per_cell_step": round(
+        sum(kb[k] * kernels[k]["count_per_step"] for k in kb) / mesh.nc, 1)}
+    cpu, accuracy = None, None
     if not args.no_cpu:
-        cpu = cpu_baseline(mesh, sc, W_start)
+        cpu, check = cpu_baseline(mesh, sc, W_start)
+        accuracy = accuracy_vs_oracle(gpu, mesh, check)
     out = {
         "metric": "Mcell-RK3-steps/s", "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_events / args.steps, 4),
@@ -423,6 +471,7 @@ def main():
                    "full_path_cells": st["full_cells"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
         "whole_step": whole_step,
