@@ -153,6 +153,8 @@ SIGNATURES = {
     "afar_gpu_evolve": ([_VP, C.c_double, C.c_double, C.c_double, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
                          C.c_int64, C.c_int64, C.c_int64], C.c_int),
     "afar_copy_field_data_device_to_host": ([P(FieldPointer), _VP, C.c_int64, C.c_int64], C.c_int),
+    "afar_adjust_time_step_noalloc": ([_VP, _VP, _VP, _VP, _VP, C.c_int64, C.c_double, C.c_double, C.c_double, _VP,
+                                       _VP], C.c_int),
     "afar_selftest_math": ([_D, _D, C.c_int32], C.c_int),
     "afar_host_math": ([_D, _D, C.c_int32], C.c_int),
     "afar_device_info": ([C.c_char_p, C.c_int32, _I32, _I64], C.c_int),

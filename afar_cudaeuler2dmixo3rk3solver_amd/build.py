@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["mesh.cpp", "precompute.cpp", "layout.cpp", "foam_io.cpp", "solver.cpp", "capi.cpp", "kernels.hip",
-           "precompute_gpu.hip"]
+           "precompute_gpu.hip", "noalloc.hip"]
 HOST_ONLY = {"mesh.cpp", "precompute.cpp", "layout.cpp", "foam_io.cpp"}  # no HIP runtime use: plain C++
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-result"]

@@ -1068,6 +1068,18 @@ int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* d
     });
 }
 
+int afar_adjust_time_step_noalloc(const double* FD, const double* CELL, const double* FACE, const int32_t* neighbour,
+                                  const double* FSHIFT, int64_t facesNum, double R, double Cv, double CFL,
+                                  double* deltaT, void* stream) {
+    return guard([&] {
+        need(FD && CELL && FACE && deltaT && (facesNum == 0 || neighbour), "NULL device array");
+        need(facesNum >= 0, "facesNum must not be negative");
+        if (facesNum > INT32_MAX) throw Error(AFAR_ERR_LIMIT, "more faces than 32-bit face ids address");
+        adjust_time_step_noalloc(FD, CELL, FACE, neighbour, FSHIFT, (int)facesNum, R, Cv, CFL, deltaT,
+                                 static_cast<hipStream_t>(stream));
+    });
+}
+
 // ---------------------------------------------------------------- diagnostics
 
 int afar_selftest_math(const double* x, double* out, int32_t n) {

@@ -371,6 +371,21 @@ int afar_gpu_evolve(afar_device* dev, double R, double Cv, double deltaT, int64_
 int afar_copy_field_data_device_to_host(afar_field_pointer* host, afar_device* dev, int64_t cellsNum,
                                         int64_t totalBoundaryFacesNum);
 
+/* adjustTimeStep without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The CFL time step
+ * over the caller's DEVICE arrays in the reference's layout -- FD 4*(nc+nb),
+ * CELL 2*(nc+nb), FACE 5*(nf+nb), neighbour 2*nf, FSHIFT 2*nf or NULL (the
+ * periodic shift of the neighbour centre) -- written to *deltaT in device
+ * memory, asynchronously on `stream` (a hipStream_t, NULL = the null
+ * stream).  Its partial minima live in an 8 KiB static device array (1024
+ * partials) per GPU, held until the process ends: calls that overlap on one
+ * GPU race; facesNum above 2^31 - 1 is refused.  Equal bit for bit to
+ * afar_adjust_time_step on the same state. */
+int afar_adjust_time_step_noalloc(const double* FD, const double* CELL, const double* FACE, const int32_t* neighbour,
+                                  const double* FSHIFT, int64_t facesNum, double R, double Cv, double CFL,
+                                  double* deltaT, void* stream);
+
 /* ------------------------------------------------------------------ diagnostics */
 /* Evaluates exp, erfc, |x|^(1/2) and 1/x of the FP64 flux path on the device: out 4*n. */
 int afar_selftest_math(const double* x, double* out, int32_t n);
