@@ -568,8 +568,7 @@ static int solver_create(const afar_mesh* h, const afar_schemes* s, const double
         cfg.exactRK = s->exactRK;
         cfg.tile = tile_of(s);
         cfg.boundaryFirstOrder = s->boundaryFirstOrder != 0;
-        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 2 || s->reconLanes == 4,
-             "reconLanes must be 0, 1, 2 or 4");
+        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 4, "reconLanes must be 0, 1 or 4");
         cfg.reconLanes = s->reconLanes;
         need(s->cellOrder >= 0 && s->cellOrder <= 2, "cellOrder must be 0, 1 or 2");
         cfg.cellOrder = s->cellOrder;
@@ -619,8 +618,7 @@ int afar_batch_create(int32_t n, const afar_mesh* const* meshes, const afar_sche
         cfg.exactRK = s->exactRK;
         cfg.tile = tile_of(s);
         cfg.boundaryFirstOrder = s->boundaryFirstOrder != 0;
-        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 2 || s->reconLanes == 4,
-             "reconLanes must be 0, 1, 2 or 4");
+        need(s->reconLanes == 0 || s->reconLanes == 1 || s->reconLanes == 4, "reconLanes must be 0, 1 or 4");
         cfg.reconLanes = s->reconLanes;
         need(s->cellOrder >= 0 && s->cellOrder <= 2, "cellOrder must be 0, 1 or 2");
         cfg.cellOrder = s->cellOrder;

@@ -55,6 +55,9 @@ struct DevPtrs {
                                // uses the geometry in float only); else null
     typename VT<R>::v4* fv;    // 2*nf reconstructed face values, [2f + side]
     R* fvT;                    // 2*nf reconstructed face temperatures (the part the BVD test reads)
+    R* tTVD;                   // 2*nf the BVD test's limited linear face temperature of that side's cell,
+                               // Tc + BJ(raw T limiter) * (grad T . d), formed by the reconstruction
+    int8_t* tlim;              // nc: BJ(raw T limiter) < VALID_LIM_FACTOR (the BVD test's limiter condition)
     // extension (faceQuadrature): flux at ngp points per internal face
     int ngp;                   // 0: face centre only (reference)
     double gpw[4];             // point weights, sum 1

@@ -177,6 +177,15 @@ __device__ __forceinline__ void limiter_process(R& lim, R& phif, R phic, R phiMa
     lim = fmin(lim, yy);
 }
 
+// Limiter transforms of K6 (cudaLimiter.h:9-32).
+template <typename R> __device__ __forceinline__ R to_bj(R y) { return fmin((R)2 * y, (R)1); }
+template <typename R> __device__ __forceinline__ R to_venkat(R y) {
+    return fmin((y * y + y + K<R>::VSMALL) / (y * y + (R)0.5 + (R)0.5 * y + K<R>::VSMALL), (R)1);
+}
+template <typename R> __device__ __forceinline__ R transform_limiter(R lim, int isShock) {
+    return (R)isShock * to_venkat(lim) + (R)(1 - isShock) * to_bj(lim);
+}
+
 // GP: the flux-quadrature extension's point states (a separate instantiation,
 // so the reference path carries none of its registers)
 template <typename R, int TILE, bool GP>
@@ -237,6 +246,8 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
     const int nf = p.nf;
     if (fabs(TMax - TMin) < K<R>::VSMALL) {
         p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
+        const R lT = to_bj((R)1);  // the BVD test's limiter of this cell
+        p.tlim[c] = lT < K<R>::VALID_LIM_FACTOR;
         for (int fi = 0; fi < C; ++fi) {
             const int sf = p.slot_face[sell(c, fi, maxC)];
             const int isNei = sf < 0;
@@ -251,6 +262,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
             o.w = wc.w + gTx * dx + gTy * dy;
             p.fv[2 * f + isNei] = o;
             p.fvT[2 * f + isNei] = o.w;
+            p.tTVD[2 * f + isNei] = wc.w + lT * (gTx * dx + gTy * dy);
             if constexpr (GP)
                 for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
                 const v4 dg = p.fgd[(size_t)g * nf + f];
@@ -333,9 +345,21 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
         }
     }
     const R nt = (R)isNotTrouble;
-    p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
+    const R limTraw = fmax(limT, (R)0) * nt;
+    p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, limTraw};
     smoothness = smoothness / ((R)0.5 * (pMax + pMin) * nt + K<R>::VSMALL);
     p.smooth[c] = smoothness > (R)1;
+    // the BVD test's inputs of this cell (cudaLimiter.h:431-440): its limiter
+    // condition and the limited linear temperature on each internal face
+    const R lT = to_bj(limTraw);
+    p.tlim[c] = lT < K<R>::VALID_LIM_FACTOR;
+    for (int fi = 0; fi < C; ++fi) {
+        const int sf = p.slot_face[sell(c, fi, maxC)];
+        const int f = (sf < 0 ? -sf : sf) - 1;
+        if (f >= nf) continue;
+        const v2 d = p.slot_d[sell(c, fi, maxC)];
+        p.tTVD[2 * f + (sf < 0)] = wc.w + lT * (gTx * d.x + gTy * d.y);
+    }
 }
 
 // Moves within groups of four consecutive lanes (DPP quad permutations, full
@@ -507,9 +531,11 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         const R gpx = quad_bcast<0>(gxk), gpy = quad_bcast<0>(gyk), guxx = quad_bcast<1>(gxk),
                 guyx = quad_bcast<1>(gyk), guxy = quad_bcast<2>(gxk), guyy = quad_bcast<2>(gyk),
                 gTx = quad_bcast<3>(gxk), gTy = quad_bcast<3>(gyk);
+        const R lT = to_bj((R)1);  // the BVD test's limiter of this cell
         if (k == 0) {
             p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
             p.smooth[c] = 0;
+            p.tlim[c] = lT < K<R>::VALID_LIM_FACTOR;
         }
         if (live) {
             v4 o;
@@ -519,6 +545,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
             o.w = wc.w + gTx * dx + gTy * dy;
             p.fv[2 * f + isNei] = o;
             p.fvT[2 * f + isNei] = o.w;
+            p.tTVD[2 * f + isNei] = wc.w + lT * (gTx * dx + gTy * dy);
             if constexpr (GP)
                 for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
                 const v4 dg = p.fgd[(size_t)g * nf + f];
@@ -624,228 +651,26 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     const int lastLane = grp ? 31 - __builtin_clz(grp) : 0;
     const int nt0 = __shfl(faceNotTrouble, base + lastLane);
     const int isNotTrouble = grp ? nt0 : 1;
+    const R nt = (R)isNotTrouble;
+    const R limTraw = fmax(limT, (R)0) * nt;
+    // the BVD test's inputs (cudaLimiter.h:431-440): the limited linear
+    // temperature on lane k's face, and the cell's limiter condition
+    const R lT = to_bj(limTraw);
+    if (live) p.tTVD[2 * f + isNei] = wc.w + lT * (gTx * dx + gTy * dy);
     if (k == 0) {
-        const R nt = (R)isNotTrouble;
-        p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
+        p.lim[c] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, limTraw};
         smoothness = smoothness / ((R)0.5 * (pMax + pMin) * nt + K<R>::VSMALL);
         p.smooth[c] = smoothness > (R)1;
-    }
-}
-
-// K5 with the face work of a tile compacted: for meshes whose cells have at
-// most four faces.  4*TILE threads per tile.
-//  phase 1  four lanes per cell: gradient and compact min/max (every lane of
-//           the group), the flat branch (its face values by lane k), and for
-//           cells on the full path the per-cell data in LDS and one work item
-//           per internal face, counted into buckets by stencil size;
-//  phase 2  the items, ordered by stencil size so a wavefront's RBF loops
-//           have similar trip counts, spread over all threads: RBF face
-//           value, limiter candidates, smoothness term, trouble test;
-//  phase 3  one lane per full-path cell reduces its faces' results in slot
-//           order (min for the limiter, max for the smoothness, the last
-//           internal face's trouble test), exactly as the sequential loop.
-// The per-face arithmetic is that of k_reconstruct, so every output equals
-// the one-lane kernel's bit for bit.  Flat and full cells, triangles and
-// quads no longer idle each other's lanes.
-template <typename R, int TILE>
-__global__ __launch_bounds__(4 * TILE) void k_reconstruct_c(DevPtrs<R> p) {
-    using v4 = typename VT<R>::v4;
-    using v2 = typename VT<R>::v2;
-    constexpr int NT = 4 * TILE, MAXC = 4, NB = kMaxSten + 1;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    v4* sW = reinterpret_cast<v4*>(smem_raw);
-    uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
-    // per-cell data of full-path cells: gradient (8), min/max (8)
-    __shared__ R sCell[TILE][16];
-    // per-slot results: limiter candidates (4), smoothness term, trouble test
-    __shared__ R sRes[TILE * MAXC][5];
-    __shared__ uint8_t sNT[TILE * MAXC];
-    __shared__ uint8_t sLive[TILE * MAXC];
-    __shared__ uint8_t sFull[TILE];
-    __shared__ uint16_t sItem[TILE * MAXC];
-    __shared__ int sBucket[NB + 1];
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    if (tile >= p.ntiles) return;
-    const int t = threadIdx.x;
-    const int hb = p.halo_ptr[tile], hn = p.halo_ptr[tile + 1] - hb;
-    if (t < TILE && tile * TILE + t < p.nc) sW[t] = p.W[tile * TILE + t];
-    for (int i = t; i < hn; i += NT) sW[TILE + i] = p.W[p.halo[hb + i]];
-    stage_stencil(sSt, p.sten, tile, TILE, p.nc, p.maxS, t, NT);
-    for (int i = t; i <= NB; i += NT) sBucket[i] = 0;
-    __syncthreads();
-
-    const int maxS = p.maxS, maxC = p.maxC, KR = maxC * (maxS + 1), nf = p.nf;
-    // ---- phase 1
-    const int lc = t >> 2, k = t & 3;
-    const int c = tile * TILE + lc;
-    const bool valid = c < p.nc;
-    int S = 0;
-    bool full = false, live = false;
-    if (valid) {
-        S = p.scnt[c];
-        const int C = p.ccnt[c];
-        const v4 wc = sW[lc];
-        R gpx = 0, gpy = 0, guxx = 0, guyx = 0, guxy = 0, guyy = 0, gTx = 0, gTy = 0;
-        for (int i = 0; i < C; ++i) {
-            const v2 M = p.lsq[sell(c, i, maxC)];
-            const v4 wj = sW[sSt[sell(lc, i, maxS)]];
-            R dphi = wj.x - wc.x;
-            gpx += M.x * dphi;
-            gpy += M.y * dphi;
-            dphi = wj.y - wc.y;
-            guxx += M.x * dphi;
-            guyx += M.y * dphi;
-            dphi = wj.z - wc.z;
-            guxy += M.x * dphi;
-            guyy += M.y * dphi;
-            dphi = wj.w - wc.w;
-            gTx += M.x * dphi;
-            gTy += M.y * dphi;
-        }
-        if (k == 0) p.gp[c] = v2{gpx, gpy};
-        if (k == 1) p.gu[c] = v2{guxx, guyx};
-        if (k == 2) p.gv[c] = v2{guxy, guyy};
-        if (k == 3) p.gT[c] = v2{gTx, gTy};
-        R pMax = wc.x, pMin = wc.x, uxMax = wc.y, uxMin = wc.y, uyMax = wc.z, uyMin = wc.z, TMax = wc.w,
-          TMin = wc.w;
-        for (int i = 0; i < C; ++i) {
-            const v4 wj = sW[sSt[sell(lc, i, maxS)]];
-            pMax = fmax(wj.x, pMax);
-            pMin = fmin(wj.x, pMin);
-            uxMax = fmax(wj.y, uxMax);
-            uxMin = fmin(wj.y, uxMin);
-            uyMax = fmax(wj.z, uyMax);
-            uyMin = fmin(wj.z, uyMin);
-            TMax = fmax(wj.w, TMax);
-            TMin = fmin(wj.w, TMin);
-        }
-        const int sf = k < C ? p.slot_face[sell(c, k, maxC)] : 0;
-        const int f = (sf < 0 ? -sf : sf) - 1;
-        live = k < C && f < nf;
-        full = !(fabs(TMax - TMin) < K<R>::VSMALL);
-        if (!full) {
-            if (k == 0) {
-                p.lim[c] = v4{(R)1, (R)1, (R)1, (R)1};
-                p.smooth[c] = 0;
-            }
-            if (live) {
-                const v2 d = p.slot_d[sell(c, k, maxC)];
-                const R dx = d.x, dy = d.y;
-                v4 o;
-                o.x = wc.x + gpx * dx + gpy * dy;
-                o.y = wc.y + guxx * dx + guyx * dy;
-                o.z = wc.z + guxy * dx + guyy * dy;
-                o.w = wc.w + gTx * dx + gTy * dy;
-                const int isNei = sf < 0;
-                p.fv[2 * f + isNei] = o;
-                p.fvT[2 * f + isNei] = o.w;
-            }
-        } else if (k == 0) {
-            R* d = sCell[lc];
-            d[0] = gpx, d[1] = gpy, d[2] = guxx, d[3] = guyx, d[4] = guxy, d[5] = guyy, d[6] = gTx, d[7] = gTy;
-            d[8] = pMax, d[9] = pMin, d[10] = uxMax, d[11] = uxMin, d[12] = uyMax, d[13] = uyMin, d[14] = TMax;
-            d[15] = TMin;
-        }
-        if (k == 0) sFull[lc] = full;
-        sLive[lc * MAXC + k] = live;
-        if (full && live) atomicAdd(&sBucket[S + 1], 1);
-    } else if (k == 0) {
-        sFull[lc] = 0;
-    }
-    if (!valid) sLive[lc * MAXC + k] = 0;
-    __syncthreads();
-    if (t == 0)  // bucket starts: items ordered by stencil size
-        for (int i = 1; i <= NB; ++i) sBucket[i] += sBucket[i - 1];
-    __syncthreads();
-    if (full && live) sItem[atomicAdd(&sBucket[S], 1)] = (uint16_t)((lc << 2) | k);
-    __syncthreads();
-    const int nItems = sBucket[NB];
-
-    // ---- phase 2
-    for (int it = t; it < nItems; it += NT) {
-        const int e = sItem[it], lci = e >> 2, ki = e & 3;
-        const int ci = tile * TILE + lci;
-        const int Si = p.scnt[ci];
-        const v4 wc = sW[lci];
-        const R* d = sCell[lci];
-        const int sf = p.slot_face[sell(ci, ki, maxC)];
-        const int isNei = sf < 0;
-        const int f = (sf < 0 ? -sf : sf) - 1;
-        const v2 dd = p.slot_d[sell(ci, ki, maxC)];
-        const R dx = dd.x, dy = dd.y;
-        const R deltaR = p.slot_dr[sell(ci, ki, maxC)];
-        const R fraction = sqrt(dx * dx + dy * dy) / deltaR;
-        const int nbl = sSt[sell(lci, ki, maxS)];
-        const R* a = p.rbf + sell(ci, ki * (maxS + 1), KR);
-        R coef = a[0];
-        R pf = coef * wc.x, uxf = coef * wc.y, uyf = coef * wc.z, Tf = coef * wc.w;
-        for (int m = 0; m < Si; ++m) {
-            const v4 wj = sW[sSt[sell(lci, m, maxS)]];
-            coef = a[(size_t)(m + 1) * 64];
-            pf += coef * wj.x;
-            uxf += coef * wj.y;
-            uyf += coef * wj.z;
-            Tf += coef * wj.w;
-        }
-        const R rP = fabs(d[8] - d[9]), rUx = fabs(d[10] - d[11]), rUy = fabs(d[12] - d[13]), rT = fabs(d[14] - d[15]);
-        R limP = 2, limUx = 2, limUy = 2, limT = 2;
-        v4 o;
-        R phif;
-        limiter_process(limP, phif, wc.x, d[8], d[9], d[0], d[1], dx, dy);
-        o.x = (R)(rP >= K<R>::SMALL) * pf + (R)(rP < K<R>::SMALL) * phif;
-        const R sm = fabs(((R)1 - fraction) * wc.x + fraction * sW[nbl].x - phif) / deltaR +
-                     (R)(pf <= (R)0) * (R)1e20 + (R)(phif <= (R)0) * (R)1e20;
-        limiter_process(limUx, phif, wc.y, d[10], d[11], d[2], d[3], dx, dy);
-        o.y = (R)(rUx >= K<R>::SMALL) * uxf + (R)(rUx < K<R>::SMALL) * phif;
-        limiter_process(limUy, phif, wc.z, d[12], d[13], d[4], d[5], dx, dy);
-        o.z = (R)(rUy >= K<R>::SMALL) * uyf + (R)(rUy < K<R>::SMALL) * phif;
-        limiter_process(limT, phif, wc.w, d[14], d[15], d[6], d[7], dx, dy);
-        o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
-        p.fv[2 * f + isNei] = o;
-        p.fvT[2 * f + isNei] = o.w;
-        R* r = sRes[lci * MAXC + ki];
-        r[0] = limP, r[1] = limUx, r[2] = limUy, r[3] = limT, r[4] = sm;
-        sNT[lci * MAXC + ki] = phif > (R)0;
-    }
-    __syncthreads();
-
-    // ---- phase 3: per-cell reductions in slot order
-    if (t < TILE && sFull[t]) {
-        const int cc = tile * TILE + t;
-        R limP = 2, limUx = 2, limUy = 2, limT = 2, smoothness = 0;
-        int isNotTrouble = 1;
-        for (int s = 0; s < MAXC; ++s) {
-            if (!sLive[t * MAXC + s]) continue;
-            const R* r = sRes[t * MAXC + s];
-            limP = fmin(limP, r[0]);
-            limUx = fmin(limUx, r[1]);
-            limUy = fmin(limUy, r[2]);
-            limT = fmin(limT, r[3]);
-            smoothness = fmax(smoothness, r[4]);
-            isNotTrouble = sNT[t * MAXC + s];
-        }
-        const R nt = (R)isNotTrouble;
-        const R* d = sCell[t];
-        p.lim[cc] = v4{fmax(limP, (R)0) * nt, fmax(limUx, (R)0) * nt, fmax(limUy, (R)0) * nt, fmax(limT, (R)0) * nt};
-        smoothness = smoothness / ((R)0.5 * (d[8] + d[9]) * nt + K<R>::VSMALL);
-        p.smooth[cc] = smoothness > (R)1;
+        p.tlim[c] = lT < K<R>::VALID_LIM_FACTOR;
     }
 }
 
 // ---------------------------------------------------------------- K6 BVD indicator
 
-template <typename R> __device__ __forceinline__ R to_bj(R y) { return fmin((R)2 * y, (R)1); }
-template <typename R> __device__ __forceinline__ R to_venkat(R y) {
-    return fmin((y * y + y + K<R>::VSMALL) / (y * y + (R)0.5 + (R)0.5 * y + K<R>::VSMALL), (R)1);
-}
-template <typename R> __device__ __forceinline__ R transform_limiter(R lim, int isShock) {
-    return (R)isShock * to_venkat(lim) + (R)(1 - isShock) * to_bj(lim);
-}
 
 template <typename R, int TILE>
 __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     using v4 = typename VT<R>::v4;
-    using v2 = typename VT<R>::v2;
     constexpr int PF = 4;  // face slots whose data is requested before the barrier
     extern __shared__ __align__(16) unsigned char smem_raw[];
     int8_t* sS = reinterpret_cast<int8_t*>(smem_raw);
@@ -856,17 +681,16 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const int maxS = p.maxS, maxC = p.maxC, nf = p.nf;
     const bool inb = c < p.nc;
     // Two rounds of independent loads before the barrier: (1) the cell's
-    // counts, flag, limiter, temperature and its gradient, the first slots'
-    // faces, halo ids and the stencil block; (2) halo flags and the slots'
-    // face temperatures and offsets.  Indices are clamped rather than
-    // branched on, and values held in plain scalars, so that no load is
-    // waited for before its round is complete.
+    // counts, flag and limiter condition, the first slots' faces, halo ids
+    // and the stencil block; (2) halo flags and the slots' face temperatures
+    // (both sides' O3 values and this side's limited linear value, formed by
+    // the reconstruction).  Indices are clamped rather than branched on, and
+    // values held in plain scalars, so that no load is waited for before its
+    // round is complete.
     const int cc = inb ? c : p.nc - 1;
     const int S = p.scnt[cc], C = inb ? (int)p.ccnt[cc] : 0;
     const int8_t own = p.smooth[cc];
-    const v4 limc = p.lim[cc];
-    const v2 gT = p.gT[cc];
-    const R Tc = p.W[cc].w;
+    const int limited = p.tlim[cc];
     int sfs[PF];
 #pragma unroll
     for (int fi = 0; fi < PF; ++fi) sfs[fi] = fi < maxC ? p.slot_face[sell(cc, fi, maxC)] : 0;
@@ -883,8 +707,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const unsigned hid1 = (unsigned)p.halo[hb + min(tid + TILE, hmax)];
     const int8_t hs0 = p.smooth[hid0];
     const int8_t hs1 = p.smooth[hid1];
-    R tfo[PF], tfn[PF];
-    R dxs[PF], dys[PF];
+    R tfo[PF], tfn[PF], tvd[PF];
 #pragma unroll
     for (int fi = 0; fi < PF; ++fi) {
         const int sf = sfs[fi];
@@ -893,10 +716,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
         const int fs = in ? f : 0;
         tfo[fi] = p.fvT[2 * fs];
         tfn[fi] = p.fvT[2 * fs + 1];
-        // a slot without an internal face reads slot 0's offset (a line read anyway)
-        const v2 d = p.slot_d[sell(cc, in ? fi : 0, maxC)];
-        dxs[fi] = d.x;
-        dys[fi] = d.y;
+        tvd[fi] = p.tTVD[2 * fs + (sf < 0)];
     }
     if (inb) sS[tid] = own;
     uint4* stDst = reinterpret_cast<uint4*>(sSt);
@@ -919,16 +739,16 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
 
     // Every face is visited; the reference leaves the loop at a cell's second
     // boundary face with bvd = 1, the largest value, so the result is the same.
+    // cudaLimiter.h:431-448, with Tf_cur_TVD = Tc + BJ(lim) * (grad T . d) and
+    // the condition lim < VALID_LIM_FACTOR taken from the reconstruction.
     int bvd = 0, bfaces = 0;
-    const R lim = to_bj(limc.w);
-    auto face = [&](int sf, R TfOwn, R TfNei, R dx, R dy) {
+    const int isLimited = limited + (shockTypeExt > 0);
+    auto face = [&](int sf, R TfOwn, R TfNei, R Tf_cur_TVD) {
         const int isNei = sf < 0;
         const R Tf_cur = (R)isNei * TfNei + (R)(1 - isNei) * TfOwn;
         const R Tf_nei = (R)(1 - isNei) * TfNei + (R)isNei * TfOwn;
-        const R Tf_cur_TVD = Tc + lim * (gT.x * dx + gT.y * dy);
         const int typeval = fabs(Tf_cur_TVD - Tf_nei) < K<R>::BVD_FACTOR * fabs(Tf_cur - Tf_nei);
         const int isNotFlat = fabs(Tf_cur - Tf_nei) > K<R>::VSMALL;
-        const int isLimited = (lim < K<R>::VALID_LIM_FACTOR) + (shockTypeExt > 0);
         bvd = max(bvd, isNotFlat * isLimited * typeval);
     };
 #pragma unroll
@@ -939,18 +759,16 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
             if (f >= nf)
                 bfaces += 1;
             else
-                face(sf, tfo[fi], tfn[fi], dxs[fi], dys[fi]);
+                face(sf, tfo[fi], tfn[fi], tvd[fi]);
         }
     }
     for (int fi = PF; fi < C; ++fi) {  // cells with more than PF faces
         const int sf = p.slot_face[sell(c, fi, maxC)];
         const int f = (sf < 0 ? -sf : sf) - 1;
-        if (f >= nf) {
+        if (f >= nf)
             bfaces += 1;
-        } else {
-            const v2 d = p.slot_d[sell(c, fi, maxC)];
-            face(sf, p.fvT[2 * f], p.fvT[2 * f + 1], d.x, d.y);
-        }
+        else
+            face(sf, p.fvT[2 * f], p.fvT[2 * f + 1], p.tTVD[2 * f + (sf < 0)]);
     }
     if (bfaces >= 2) bvd = 1;  // corner cells use the second-order reconstruction
     int finalType = (shockType > 0) * (shockTypeExt > 1) * (-2) + (bvd > 0);
@@ -1406,13 +1224,6 @@ void launch_reconstruct_t(const DevPtrs<R>& p, int nb, int tile, size_t lds, hip
 template <typename R>
 void launch_reconstruct(const DevPtrs<R>& p, int tile, size_t lds, hipStream_t s, int lanes) {
     const int nb = pad8(p.ntiles);
-    if (lanes == 2) {
-        if (tile == 64)
-            hipLaunchKernelGGL((k_reconstruct_c<R, 64>), dim3(nb), dim3(256), lds, s, p);
-        else
-            hipLaunchKernelGGL((k_reconstruct_c<R, 128>), dim3(nb), dim3(512), lds, s, p);
-        return;
-    }
     if (p.ngp > 0) {
         launch_reconstruct_t<R, true>(p, nb, tile, lds, s, lanes);
         return;
@@ -1478,8 +1289,7 @@ hipError_t set_lds_limits(size_t lds) {
                          (const void*)k_reconstruct_q<R, 128, false>, (const void*)k_reconstruct_q<R, 256, false>,
                          (const void*)k_reconstruct<R, 64, true>,     (const void*)k_reconstruct<R, 128, true>,
                          (const void*)k_reconstruct<R, 256, true>,    (const void*)k_reconstruct_q<R, 64, true>,
-                         (const void*)k_reconstruct_q<R, 128, true>,  (const void*)k_reconstruct_q<R, 256, true>,
-                         (const void*)k_reconstruct_c<R, 64>,         (const void*)k_reconstruct_c<R, 128>};
+                         (const void*)k_reconstruct_q<R, 128, true>,  (const void*)k_reconstruct_q<R, 256, true>};
     for (const void* fn : fns) {
         // the dynamic limit plus the kernel's static LDS may not exceed the block limit
         hipFuncAttributes attr;

@@ -72,10 +72,7 @@ class SolverImpl final : public DeviceSolver {
         // LDS of the reconstruction: tile + halo states, then the tile's stencil ids
         lds_recon_ = (size_t)(L_.tile + L_.maxHalo) * sizeof(v4) + (size_t)L_.tile * L_.maxS * sizeof(uint16_t);
         lds_m_off_ = (lds_recon_ + 15) & ~(size_t)15;
-        if ((cfg.reconLanes == 4 || cfg.reconLanes == 2) && L_.maxC > 4)
-            throw Error(1, "reconLanes 2 and 4 need cells with at most four faces");
-        if (cfg.reconLanes == 2 && L_.tile > 128) throw Error(1, "reconLanes 2 needs a tile of at most 128 cells");
-        if (cfg.reconLanes == 2 && ra.ngp) throw Error(1, "reconLanes 2 does not support faceQuadrature");
+        if (cfg.reconLanes == 4 && L_.maxC > 4) throw Error(1, "reconLanes 4 needs cells with at most four faces");
         if (cfg.reconLanes != 0)
             lanes_ = cfg.reconLanes;
         else
@@ -90,13 +87,9 @@ class SolverImpl final : public DeviceSolver {
         hipDeviceProp_t prop;
         check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
         size_t lds_cap = prop.sharedMemPerBlock;
-        // the compacted kernel adds fixed per-tile arrays (k_reconstruct_c)
-        const size_t lds_static = lanes_ == 2 ? (size_t)L_.tile * (16 + 20) * sizeof(R) + (size_t)L_.tile * 13 +
-                                                    (kMaxSten + 2) * sizeof(int)
-                                              : 0;
-        if (lds_recon_ + lds_static > lds_cap)
+        if (lds_recon_ > lds_cap)
             throw Error(3, "Too large local block stencil size: reconstruct needs " +
-                               std::to_string(lds_recon_ + lds_static) + " bytes of LDS, the device allows " +
+                               std::to_string(lds_recon_) + " bytes of LDS, the device allows " +
                                std::to_string(lds_cap));
         // The reconstruction kernels' dynamic-LDS limit is raised to the
         // device's per-block maximum once per process: the attribute is
@@ -576,6 +569,9 @@ class SolverImpl final : public DeviceSolver {
         m_flat_prev.alloc(nc);
         P_.flat_prev = m_flat_prev.as<uint8_t>();
         check_hip(hipMemset(P_.flat_prev, 0, nc), "hipMemset");
+        m_tlim.alloc(nc);
+        P_.tlim = m_tlim.as<int8_t>();
+        check_hip(hipMemset(P_.tlim, 0, nc), "hipMemset");
         P_.scnt = put(m_scnt, L_.scnt.data(), nc);
         P_.ccnt = put(m_ccnt, L_.ccnt.data(), nc);
         P_.sten = put(m_sten, L_.sten.data(), L_.sten.size());
@@ -617,6 +613,8 @@ class SolverImpl final : public DeviceSolver {
         P_.fv = m_fv.as<v4>();
         m_fvT.alloc(sizeof(R) * 2 * nf);
         P_.fvT = m_fvT.as<R>();
+        m_tTVD.alloc(sizeof(R) * 2 * nf);
+        P_.tTVD = m_tTVD.as<R>();
         P_.ngp = L_.ngp;
         if (L_.ngp > 4) throw Error(3, "more than four flux quadrature points");
         for (int g = 0; g < L_.ngp; ++g) P_.gpw[g] = L_.gpw[g];
@@ -698,7 +696,7 @@ class SolverImpl final : public DeviceSolver {
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_slot_dr, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fgeo32,
         m_fv, m_flux, m_flux32, m_bcell, m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res,
-        m_rbf_gp, m_fgd, m_fvg, m_flat_prev;
+        m_rbf_gp, m_fgd, m_fvg, m_flat_prev, m_tlim, m_tTVD;
 };
 
 // A solver on relabelled cells and faces (compact_tile_order) that reads and

@@ -18,7 +18,7 @@ struct SolverConfig {
     int exactRK = 0;
     int tile = 64;
     int boundaryFirstOrder = 0;
-    int reconLanes = 0;  // reconstruction kernel: 0 automatic, 1 one lane per cell, 2 compacted, 4 four lanes
+    int reconLanes = 0;  // reconstruction kernel: 0 automatic, 1 one lane per cell, 4 four lanes
     int cellOrder = 0;   // device cell order: 0 compact tiles (Hilbert curve) with blocks of 32
                          // (FP64) or 64 (mixed) cells ordered by face count and stencil size,
                          // 1 the mesh's order, 2 compact tiles in Hilbert order throughout

@@ -71,10 +71,13 @@ def kernel_bytes(st, w=8, ngp=0):
     nc, nf, nb, sS, sC = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"]
     sR, sfull = st["sum_int_coef_full"], st["int_slots_full"]
     ns = 2 * nf  # internal face slots
+    # writes: gradients 8w, limiters 4w, smoothness flag and limiter condition per cell; O3 face state 4w,
+    # its temperature w and the BVD test's limited linear temperature w per internal face side
     recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * sfull + 2 * w * sC + w * sR
-             + 8 * w * nc + 4 * w * nc + nc + 4 * w * ns + w * ns)
-    # the limiter transform is applied by the face pass: bvd writes only the types
-    bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * ns + 4 * w * nc + 2 * w * nc + w * nc + nc)
+             + 8 * w * nc + 4 * w * nc + 2 * nc + 4 * w * ns + w * ns + w * ns)
+    # bvd reads flags, counts, stencil ids, face slots, both sides' O3 face temperatures and this side's
+    # limited linear one, the limiter condition; writes the types (the face pass applies the limiter transform)
+    bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + w * ns + w * ns + nc + nc)
     # FP32-mixed: internal fluxes at the face centre are stored as floats, and
     # the flux pass without the CFL bound reads the face geometry in float
     fw = 4 * w if (w == 4 and not ngp) else 32
@@ -264,9 +267,8 @@ def main():
     ap.add_argument("--t-start", type=float, default=None,
                     help="simulated time reached (untimed) before the warm-up; default 0.15 for the Riemann case "
                          "(mid-run of t = 0.3, when the waves cover much of the domain), 0 otherwise")
-    ap.add_argument("--recon-lanes", type=int, default=0, choices=[0, 1, 2, 4],
-                    help="reconstruction kernel: 1 one thread per cell, 4 four per cell, 2 four per cell with "
-                         "the face work compacted; 0 automatic")
+    ap.add_argument("--recon-lanes", type=int, default=0, choices=[0, 1, 4],
+                    help="reconstruction kernel: 1 one thread per cell, 4 four per cell; 0 automatic")
     ap.add_argument("--cell-order", type=int, default=0, choices=[0, 1, 2],
                     help="device cell order: 0 compact tiles along a Hilbert curve with 32 (FP64) or 64 (mixed) "
                          "cell blocks ordered by face count and stencil size, 1 the mesh's (RCM) order, "

@@ -76,9 +76,8 @@ typedef struct afar_schemes {
                                    use first-order face states (the reference does this only for
                                    cells with two boundary faces, cudaLimiter.h:419-429) */
     int32_t reconLanes;  /* reconstruction kernel, same results: 0 automatic, 1 one thread per
-                            cell, 4 four threads per cell, 2 four threads per cell with the
-                            tile's face work compacted (2 and 4: cells of at most four faces;
-                            2: tiles of at most 128 cells; automatic picks 4, else 1) */
+                            cell, 4 four threads per cell (cells of at most four faces;
+                            automatic picks 4, else 1) */
     int32_t cellOrder;   /* device order of cells, same results: 0 compact tiles (cells along a
                             Hilbert curve through their centres, faces by owner) whose blocks of
                             32 (FP64) or 64 (mixed) cells are ordered by face count and stencil
