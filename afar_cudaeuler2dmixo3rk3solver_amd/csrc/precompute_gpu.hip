@@ -1,10 +1,10 @@
-// RBF precompute on the GPU (basicCUDAsolver.cpp:461-619): one thread per
+// RBF precompute on the GPU (basicCUDAsolver.cpp:461-619): one wavefront per
 // cell assembles the moment matrix from the members' quadrature, inverts it
 // by the Jacobi SVD and writes the coefficients of the cell's side of each of
 // its internal faces.  The arithmetic is rbf_core.h, shared with the host
 // precompute, so the coefficients equal the host's bit for bit.  Stencils,
 // quadrature tables and geometry come from the host; the work matrices (three
-// of at most 21 x 21 doubles per thread) live in private memory.
+// of at most 21 x 21 doubles per cell) live in LDS.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -43,46 +43,137 @@ struct RbfArgs {
     double* rbf_gp;
 };
 
-// Few waves per SIMD: each thread holds about 11 KB of private work arrays.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void k_rbf_precompute(RbfArgs a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.nc) return;
-    const int S = a.S[c], n = S + 1;
-    double X[2 * kRbfMaxN], U[kRbfMaxN * kRbfMaxN], V[kRbfMaxN * kRbfMaxN], G[kRbfMaxN * kRbfMaxN];
-    double sig[kRbfMaxN], b[kRbfMaxN], out[kRbfMaxN];
-    int id[kRbfMaxN], count[kRbfMaxN];
-    double shx[kRbfMaxN], shy[kRbfMaxN];
-    X[0] = a.C[c].x;
-    X[1] = a.C[c].y;
-    id[0] = c;
-    shx[0] = shy[0] = 0.0;
-    for (int i = 1; i < n; ++i) {
-        const size_t s = (size_t)c * kMaxValidStencil + i - 1;
-        id[i] = a.sten[s];
-        shx[i] = a.sten_shift[s].x;
-        shy[i] = a.sten_shift[s].y;
-        X[2 * i] = a.C[id[i]].x + shx[i];
-        X[2 * i + 1] = a.C[id[i]].y + shy[i];
+// One wavefront per cell (a block of 64 threads), its work matrices in LDS.
+// Every value is formed by one lane with the host's operations in the host's
+// order (rbf_core.h); the lanes share the independent parts: the moment
+// entries, the disjoint rotations of a round-robin round and their column
+// updates, the pseudo-inverse entries, and per face the basis values and
+// coefficients.
+__global__ __launch_bounds__(64) void k_rbf_precompute(RbfArgs a) {
+    constexpr int N = kRbfMaxN;
+    __shared__ double sU[N * N], sV[N * N], sG[N * N];
+    __shared__ double sX[2 * N], sShx[N], sShy[N], sSig[N], sB[N], sOut[N], sCs[2 * (N + 1) / 2 * 2];
+    __shared__ int sId[N], sCount[N], sDo[(N + 1) / 2];
+    __shared__ double sRef, sSum;
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int S = a.S[c], n = S + 1, nn = n * n;
+    if (lane < n) {
+        int id = c;
+        double shx = 0.0, shy = 0.0;
+        if (lane > 0) {
+            const size_t s = (size_t)c * kMaxValidStencil + lane - 1;
+            id = a.sten[s];
+            shx = a.sten_shift[s].x;
+            shy = a.sten_shift[s].y;
+        }
+        sId[lane] = id;
+        sShx[lane] = shx;
+        sShy[lane] = shy;
+        sX[2 * lane] = lane > 0 ? a.C[id].x + shx : a.C[c].x;
+        sX[2 * lane + 1] = lane > 0 ? a.C[id].y + shy : a.C[c].y;
+        sCount[lane] = (int)(a.tptr[id + 1] - a.tptr[id]);
     }
-    for (int i = 0; i < n; ++i) count[i] = (int)(a.tptr[id[i] + 1] - a.tptr[id[i]]);
-    double ref = 0.0;
-    for (int i = 1; i < n; ++i) {
-        const double refdx = X[2 * i] - a.C[c].x, refdy = X[2 * i + 1] - a.C[c].y;
-        ref += sqrt(refdx * refdx + refdy * refdy);
+    __syncthreads();
+    if (lane == 0) {  // mean distance to the members, summed in order
+        double ref = 0.0;
+        for (int i = 1; i < n; ++i) {
+            const double refdx = sX[2 * i] - a.C[c].x, refdy = sX[2 * i + 1] - a.C[c].y;
+            ref += sqrt(refdx * refdx + refdy * refdy);
+        }
+        ref /= (double)S;
+        sRef = ref;
+        a.ref[c] = ref;
     }
-    ref /= (double)S;
-    a.ref[c] = ref;
-    rbf_moments(n, X, count,
-                [&](int i, int k, double& x, double& y, double& w) {
-                    const int64_t e = a.tptr[id[i]] + k;
-                    x = i > 0 ? a.tp[2 * e] + shx[i] : a.tp[2 * e];
-                    y = i > 0 ? a.tp[2 * e + 1] + shy[i] : a.tp[2 * e + 1];
-                    w = a.tw[e];
-                },
-                ref, a.sf, U);
-    rbf_pinv(n, U, V, sig, G);
-    const bool trouble = rbf_trouble(n, G);
-    a.trouble[c] = trouble;
+    __syncthreads();
+    const double ref = sRef, sf = a.sf;
+    // moment entries P_ij, stored transposed (U[j*n + i] = P_ij) as rbf_pinv works on them
+    for (int e = lane; e < nn; e += 64) {
+        const int i = e / n, j = e - i * n;
+        const double x0 = sX[2 * j], y0 = sX[2 * j + 1];
+        const int64_t e0 = a.tptr[sId[i]];
+        double acc = 0.0;
+        for (int k = 0; k < sCount[i]; ++k) {
+            const double gx = i > 0 ? a.tp[2 * (e0 + k)] + sShx[i] : a.tp[2 * (e0 + k)];
+            const double gy = i > 0 ? a.tp[2 * (e0 + k) + 1] + sShy[i] : a.tp[2 * (e0 + k) + 1];
+            const double dx = (gx - x0) / ref, dy = (gy - y0) / ref;
+            const double r2 = dx * dx + dy * dy;
+            acc += a.tw[e0 + k] * sqrt(sf + r2);
+        }
+        sU[j * n + i] = acc;
+        sV[e] = (i == j) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    // Jacobi sweeps in the round-robin order: lane i < m/2 finds pair i's
+    // rotation, then all lanes apply the round's rotations to U and V columns.
+    const int m = n + (n & 1), half = m / 2;
+    for (int sweep = 0; sweep < 80; ++sweep) {
+        bool rotated = false;
+        for (int r = 0; r < m - 1; ++r) {
+            if (lane < half) {
+                int p, q;
+                rbf_rr_pair(m, r, lane, p, q);
+                double cs = 0.0, sn = 0.0;
+                const bool d = q < n && rbf_rotation(n, &sU[p * n], &sU[q * n], cs, sn);
+                sDo[lane] = d;
+                sCs[2 * lane] = cs;
+                sCs[2 * lane + 1] = sn;
+            }
+            __syncthreads();
+            bool any = false;
+            for (int i = 0; i < half; ++i) any = any || sDo[i];
+            rotated = rotated || any;
+            for (int e = lane; e < half * 2 * n; e += 64) {
+                const int i = e / (2 * n), rem = e - i * 2 * n;
+                if (!sDo[i]) continue;
+                int p, q;
+                rbf_rr_pair(m, r, i, p, q);
+                double* M = rem < n ? sU : sV;
+                const int k = rem < n ? rem : rem - n;
+                rbf_rotate(M[p * n + k], M[q * n + k], sCs[2 * i], sCs[2 * i + 1]);
+            }
+            __syncthreads();
+        }
+        if (!rotated) break;
+    }
+    if (lane < n) {
+        double s2 = 0.0;
+        for (int k = 0; k < n; ++k) s2 += sU[lane * n + k] * sU[lane * n + k];
+        sSig[lane] = sqrt(s2);
+    }
+    __syncthreads();
+    bool bad = false;
+    for (int e = lane; e < nn; e += 64) {
+        const int i = e / n, j = e - i * n;
+        const double g = rbf_pinv_entry(n, sU, sV, sSig, i, j);
+        sG[e] = g;
+        bad = bad || fabs(g) > 1e10 || g != g;
+    }
+    const bool trouble = __any(bad);
+    if (lane == 0) a.trouble[c] = trouble;
+    __syncthreads();
+    // coefficients of the cell's side of each of its internal faces
+    auto face = [&](double xf, double yf, double* o) {
+        if (lane < n) {
+            const double dx = (xf - sX[2 * lane]) / ref, dy = (yf - sX[2 * lane + 1]) / ref;
+            sB[lane] = sqrt(sf + (dx * dx + dy * dy));
+        }
+        __syncthreads();
+        if (lane < n) {
+            double v = 0.0;
+            for (int j = 0; j < n; ++j) v += sG[j * n + lane] * sB[j];
+            sOut[lane] = v;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            double sum = 0.0;
+            for (int i = 0; i < n; ++i) sum += sOut[i];
+            sSum = sum;
+        }
+        __syncthreads();
+        if (lane < n) o[lane] = (sOut[lane] - sSum / (double)n) + 1.0 / (double)n;
+        __syncthreads();
+    };
     for (int fi = 0; fi < a.Cc[c]; ++fi) {
         const int sfc = a.cellFaces[(size_t)c * kMaxValidStencil + fi];
         const int f = (sfc > 0 ? sfc : -sfc) - 1;
@@ -90,12 +181,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
         const int side = sfc < 0;
         double* o = a.rbf + (size_t)2 * a.W * f + (size_t)side * a.W;
         if (trouble) {
-            o[0] = 1.0;
-            for (int i = 1; i < n; ++i) o[i] = 0.0;
+            if (lane < n) o[lane] = lane == 0 ? 1.0 : 0.0;
             for (int g = 0; g < a.ngp; ++g) {
                 double* og = a.rbf_gp + ((size_t)g * a.nf + f) * 2 * a.W + (size_t)side * a.W;
-                og[0] = 1.0;
-                for (int i = 1; i < n; ++i) og[i] = 0.0;
+                if (lane < n) og[lane] = lane == 0 ? 1.0 : 0.0;
             }
             continue;
         }
@@ -104,8 +193,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
             xf = xf - a.fshift[f].x;
             yf = yf - a.fshift[f].y;
         }
-        rbf_face(n, G, X, xf, yf, ref, a.sf, b, out);
-        for (int i = 0; i < n; ++i) o[i] = out[i];
+        face(xf, yf, o);
         for (int g = 0; g < a.ngp; ++g) {  // the host's face_gauss_point, same operations
             const double t = a.xi[g];
             double xg = 0.5 * a.fp0[f].x * (1 - t) + 0.5 * a.fp1[f].x * (1 + t);
@@ -114,9 +202,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 4))) void
                 xg = xg - a.fshift[f].x;
                 yg = yg - a.fshift[f].y;
             }
-            rbf_face(n, G, X, xg, yg, ref, a.sf, b, out);
-            double* og = a.rbf_gp + ((size_t)g * a.nf + f) * 2 * a.W + (size_t)side * a.W;
-            for (int i = 0; i < n; ++i) og[i] = out[i];
+            face(xg, yg, a.rbf_gp + ((size_t)g * a.nf + f) * 2 * a.W + (size_t)side * a.W);
         }
     }
 }
@@ -190,7 +276,7 @@ void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
     check_hip(hipMemsetAsync(a.rbf, 0, sizeof(double) * 2 * W * (size_t)nf, s), "hipMemsetAsync");
     if (pc.ngp)
         check_hip(hipMemsetAsync(a.rbf_gp, 0, sizeof(double) * pc.ngp * 2 * W * (size_t)nf, s), "hipMemsetAsync");
-    hipLaunchKernelGGL(k_rbf_precompute, dim3((nc + 63) / 64), dim3(64), 0, s, a);
+    hipLaunchKernelGGL(k_rbf_precompute, dim3(nc), dim3(64), 0, s, a);
     check_hip(hipGetLastError(), "k_rbf_precompute");
     pc.rbf.assign((size_t)2 * W * nf, 0.0);
     pc.trouble.assign(nc, 0);

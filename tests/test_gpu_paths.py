@@ -163,6 +163,41 @@ def test_gpu_precompute_equals_host(afar, kind):
         assert np.array_equal(x, y)
 
 
+def test_gpu_precompute_against_numpy_restatement(afar):
+    """The GPU precompute against oracle/precompute_ref.py (numpy/LAPACK
+    pseudo-inverse, written independently of the library): every face side's
+    coefficients within 1e-7, constants reproduced (sum 1 to 1e-12), the
+    reference distances equal to the numpy mean distance."""
+    from oracle import precompute_ref as PR
+    say("case: GPU precompute vs the numpy restatement, riemann 14x14 (triangles and quads)")
+    m = afar.Mesh.riemann(n=14, jitter=0.2, seed=11, split_p=0.5)
+    sc = afar.schemes(cOrder=7, fOrder=2)
+    m.precompute(sc, gpu=True)
+    s = m.sizes()
+    nc, nf, nb = s["nc"], s["nf"], s["nb"]
+    S, Cc, st, cf = m.stencils()
+    rbf, trouble, refd = m.rbf_basis()
+    xy, _ = m.cells()
+    f = m.faces()
+    gauss = {k: m.gauss_points(k, 7 if k < nc else 2) for k in range(nc + nb)}
+    assert trouble.sum() == 0
+    worst = 0.0
+    for fc in range(nf):
+        for side, cell in ((0, f["own"][fc]), (1, f["nei"][fc])):
+            a, tr, _ = PR.rbf_coefficients(cell, list(st[cell, : S[cell]]), xy, gauss, f["face"][fc, :2], 10.0)
+            assert not tr
+            got = rbf[fc, side, : S[cell] + 1]
+            worst = max(worst, float(np.abs(got - a).max()))
+            assert abs(got.sum() - 1.0) < 1e-12
+            assert np.all(rbf[fc, side, S[cell] + 1:] == 0.0)
+    say("  largest coefficient difference %.2e" % worst)
+    assert worst < 1e-7
+    X = xy[:nc]
+    for c in range(nc):
+        mem = xy[st[c, : S[c]]]
+        assert abs(refd[c] - np.mean(np.hypot(mem[:, 0] - X[c, 0], mem[:, 1] - X[c, 1]))) < 1e-14
+
+
 @pytest.mark.parametrize("kind", ["riemann", "periodic"])
 def test_gpu_precompute_face_quadrature(afar, kind):
     """With flux quadrature points (faceQuadrature 1, fOrder 4: three points)
