@@ -351,10 +351,7 @@ int afar_mesh_precompute_gpu(afar_mesh* h, const afar_schemes* s) {
         Schemes st_only = sc;
         st_only.recalcBasis = 0;  // stencils on the host, coefficients on the GPU
         Precomputed pc = precompute(h->m, st_only);
-        {
-            std::lock_guard<std::mutex> lk(runtime_mutex());  // device allocations and frees
-            precompute_rbf_gpu(h->m, sc, pc);
-        }
+        precompute_rbf_gpu(h->m, sc, pc);  // stream-ordered buffers on a stream of its own
         h->pc = std::move(pc);
         finish_precompute(h, sc);
     });

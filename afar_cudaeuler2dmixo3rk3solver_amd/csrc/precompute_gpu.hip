@@ -207,17 +207,32 @@ __global__ __launch_bounds__(64) void k_rbf_precompute(RbfArgs a) {
     }
 }
 
+// The precompute's own stream, destroyed after the buffers below (declared
+// after it) have enqueued their frees.
+struct OwnStream {
+    hipStream_t s = nullptr;
+    OwnStream() { check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"); }
+    ~OwnStream() {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+};
+
+// Device buffer allocated, filled and freed in stream order (a plain hipFree
+// would wait for every other solver's work on the device).
 struct Buf {
     void* p = nullptr;
-    Buf() = default;
+    hipStream_t s = nullptr;
+    explicit Buf(hipStream_t st) : s(st) {}
     Buf(const Buf&) = delete;
     Buf& operator=(const Buf&) = delete;
     ~Buf() {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, s);
     }
+    // the host source must stay alive until the stream is synchronised
     template <typename T> T* put(const T* src, size_t n) {
-        check_hip(hipMalloc(&p, sizeof(T) * (n ? n : 1)), "hipMalloc");
-        if (n && src) check_hip(hipMemcpy(p, src, sizeof(T) * n, hipMemcpyHostToDevice), "upload");
+        check_hip(hipMallocAsync(&p, sizeof(T) * (n ? n : 1), s), "hipMallocAsync");
+        if (n && src) check_hip(hipMemcpyAsync(p, src, sizeof(T) * n, hipMemcpyHostToDevice, s), "upload");
         return static_cast<T*>(p);
     }
 };
@@ -241,7 +256,11 @@ void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
     }
     if (pc.ngp > 4) throw Error(3, "more than four flux quadrature points");
     for (size_t i = 0; i < shift.size(); ++i) shift[i] = double2{st.sten_shift[i].x, st.sten_shift[i].y};
-    Buf bS, bCc, bSten, bShift, bCF, bC, bTp, bTw, bTptr, bCf, bFsh, bRbf, bTr, bRef, bP0, bP1, bGp;
+    // own stream: waits for this work only, not for other solvers on the device
+    OwnStream os;
+    hipStream_t s = os.s;
+    Buf bS(s), bCc(s), bSten(s), bShift(s), bCF(s), bC(s), bTp(s), bTw(s), bTptr(s), bCf(s), bFsh(s), bRbf(s),
+        bTr(s), bRef(s), bP0(s), bP1(s), bGp(s);
     RbfArgs a{};
     a.nc = nc;
     a.nf = nf;
@@ -266,13 +285,6 @@ void precompute_rbf_gpu(const Mesh& m, const Schemes& sc, Precomputed& pc) {
     a.fp0 = pc.ngp ? bP0.put(p0.data(), p0.size()) : nullptr;
     a.fp1 = pc.ngp ? bP1.put(p1.data(), p1.size()) : nullptr;
     a.rbf_gp = pc.ngp ? bGp.put<double>(nullptr, (size_t)pc.ngp * 2 * W * nf) : nullptr;
-    // own stream: waits for this work only, not for other solvers on the device
-    hipStream_t s = nullptr;
-    check_hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } guard{s};
     check_hip(hipMemsetAsync(a.rbf, 0, sizeof(double) * 2 * W * (size_t)nf, s), "hipMemsetAsync");
     if (pc.ngp)
         check_hip(hipMemsetAsync(a.rbf_gp, 0, sizeof(double) * pc.ngp * 2 * W * (size_t)nf, s), "hipMemsetAsync");

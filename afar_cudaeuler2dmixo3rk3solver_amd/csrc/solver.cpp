@@ -25,20 +25,41 @@ std::mutex& runtime_mutex() {
 
 namespace {
 
+// Device buffer, allocated and freed in stream order on its solver's stream
+// (a plain hipFree waits for the whole device, so tearing one solver down
+// would stall every other solver's work).  The owner synchronises the stream
+// before the buffer is used elsewhere.
 struct DevMem {
     void* p = nullptr;
     size_t bytes = 0;
+    hipStream_t s = nullptr;
     DevMem() = default;
     DevMem(const DevMem&) = delete;
     DevMem& operator=(const DevMem&) = delete;
     ~DevMem() {
-        if (p) (void)hipFree(p);
+        if (p) (void)hipFreeAsync(p, s);
     }
-    void alloc(size_t n) {
+    void alloc(size_t n, hipStream_t st) {
+        if (p) check_hip(hipFreeAsync(p, s), "hipFreeAsync");
+        p = nullptr;
         bytes = n;
-        check_hip(hipMalloc(&p, n ? n : 16), "hipMalloc");
+        s = st;
+        check_hip(hipMallocAsync(&p, n ? n : 16, st), "hipMallocAsync");
     }
     template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// The solver's stream: created here unless the caller gave one; destroyed
+// after the buffers (members declared after it) have enqueued their frees.
+struct StreamHolder {
+    hipStream_t s = nullptr;
+    bool own = false;
+    ~StreamHolder() {
+        if (s) {
+            (void)hipStreamSynchronize(s);
+            if (own) (void)hipStreamDestroy(s);
+        }
+    }
 };
 
 template <typename R>
@@ -61,11 +82,12 @@ class SolverImpl final : public DeviceSolver {
         }
         L_ = make_dev_layout(ra, cfg.tile);
         if (cfg.stream) {
-            stream_ = cfg.stream;  // the caller's stream: used, never destroyed
+            sh_.s = cfg.stream;  // the caller's stream: used, never destroyed
         } else {
-            check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
-            own_stream_ = true;
+            check_hip(hipStreamCreateWithFlags(&sh_.s, hipStreamNonBlocking), "hipStreamCreate");
+            sh_.own = true;
         }
+        stream_ = sh_.s;
         check_hip(hipEventCreate(&ev0_), "hipEventCreate");
         check_hip(hipEventCreate(&ev1_), "hipEventCreate");
         for (hipEvent_t& e : evq_) check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
@@ -104,10 +126,7 @@ class SolverImpl final : public DeviceSolver {
     ~SolverImpl() override {
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
-        if (stream_) {
-            (void)hipStreamSynchronize(stream_);
-            if (own_stream_) (void)hipStreamDestroy(stream_);
-        }
+        if (stream_) (void)hipStreamSynchronize(stream_);  // the buffers' frees follow, in stream order
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
         for (hipEvent_t e : evq_)
@@ -424,14 +443,20 @@ class SolverImpl final : public DeviceSolver {
         synchronize();
         if (P_.flux32 && strcmp(what, "flux") == 0) {  // internal faces from their float store
             std::vector<float4> q(nf);
-            if (nf) check_hip(hipMemcpy(q.data(), P_.flux32, sizeof(float4) * nf, hipMemcpyDeviceToHost), "debug copy");
+            if (nf)
+                check_hip(hipMemcpyAsync(q.data(), P_.flux32, sizeof(float4) * nf, hipMemcpyDeviceToHost, stream_),
+                          "debug copy");
+            synchronize();
             double4* o = static_cast<double4*>(out);
             for (size_t f = 0; f < nf; ++f) o[f] = double4{q[f].x, q[f].y, q[f].z, q[f].w};
             if (nb)
-                check_hip(hipMemcpy(o + nf, P_.flux + nf, sizeof(double4) * nb, hipMemcpyDeviceToHost), "debug copy");
+                check_hip(hipMemcpyAsync(o + nf, P_.flux + nf, sizeof(double4) * nb, hipMemcpyDeviceToHost, stream_),
+                          "debug copy");
+            synchronize();
             return;
         }
-        check_hip(hipMemcpy(out, src, bytes, hipMemcpyDeviceToHost), "debug copy");
+        check_hip(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, stream_), "debug copy");
+        synchronize();
     }
 
     void run_stage_kernels(int upto, double dt, int stage) override {
@@ -512,9 +537,13 @@ class SolverImpl final : public DeviceSolver {
         check_hip(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
     }
 
+    void dalloc(DevMem& m, size_t bytes) { m.alloc(bytes, stream_); }
+    void dzero(void* p, size_t bytes) { check_hip(hipMemsetAsync(p, 0, bytes, stream_), "hipMemsetAsync"); }
+    // allocation and upload in stream order; returns once src may be reused
     template <typename T> T* put(DevMem& m, const T* src, size_t n) {
-        m.alloc(sizeof(T) * n);
-        if (n && src) check_hip(hipMemcpy(m.p, src, sizeof(T) * n, hipMemcpyHostToDevice), "upload");
+        dalloc(m, sizeof(T) * n);
+        if (n && src) check_hip(hipMemcpyAsync(m.p, src, sizeof(T) * n, hipMemcpyHostToDevice, stream_), "upload");
+        synchronize();
         return m.as<T>();
     }
 
@@ -527,10 +556,12 @@ class SolverImpl final : public DeviceSolver {
             w[i].z = (R)ra_.FD[4 * i + 2];
             w[i].w = (R)ra_.FD[4 * i + 3];
         }
+        check_hip(hipMemcpyAsync(P_.W, w.data(), sizeof(v4) * (nc + nb), hipMemcpyHostToDevice, stream_), "W upload");
+        check_hip(hipMemcpyAsync(P_.U, ra_.cFD.data(), sizeof(double4) * nc, hipMemcpyHostToDevice, stream_),
+                  "U upload");
+        check_hip(hipMemcpyAsync(P_.Uold, ra_.cFD.data(), sizeof(double4) * nc, hipMemcpyHostToDevice, stream_),
+                  "Uold upload");
         synchronize();
-        check_hip(hipMemcpy(P_.W, w.data(), sizeof(v4) * (nc + nb), hipMemcpyHostToDevice), "W upload");
-        check_hip(hipMemcpy(P_.U, ra_.cFD.data(), sizeof(double4) * nc, hipMemcpyHostToDevice), "U upload");
-        check_hip(hipMemcpy(P_.Uold, ra_.cFD.data(), sizeof(double4) * nc, hipMemcpyHostToDevice), "Uold upload");
     }
 
     void upload() {
@@ -546,32 +577,32 @@ class SolverImpl final : public DeviceSolver {
         P_.lds_m_off = (int)lds_m_off_;
         P_.lds_bvd_sten_off = (int)lds_bvd_sten_off_;
         P_.boundaryFirstOrder = cfg_.boundaryFirstOrder;
-        m_W.alloc(sizeof(v4) * (nc + nb));
+        dalloc(m_W, sizeof(v4) * (nc + nb));
         P_.W = m_W.as<v4>();
-        m_U.alloc(sizeof(double4) * nc);
+        dalloc(m_U, sizeof(double4) * nc);
         P_.U = m_U.as<double4>();
-        m_Uold.alloc(sizeof(double4) * nc);
+        dalloc(m_Uold, sizeof(double4) * nc);
         P_.Uold = m_Uold.as<double4>();
         P_.vol = put(m_vol, L_.vol.data(), nc);
-        for (DevMem* m : {&m_gp, &m_gu, &m_gv, &m_gT}) m->alloc(sizeof(v2) * nc);
+        for (DevMem* m : {&m_gp, &m_gu, &m_gv, &m_gT}) dalloc(*m, sizeof(v2) * nc);
         P_.gp = m_gp.as<v2>();
         P_.gu = m_gu.as<v2>();
         P_.gv = m_gv.as<v2>();
         P_.gT = m_gT.as<v2>();
-        m_lim.alloc(sizeof(v4) * nc);
+        dalloc(m_lim, sizeof(v4) * nc);
         P_.lim = m_lim.as<v4>();
-        m_smooth.alloc(nc + nb);
+        dalloc(m_smooth, nc + nb);
         P_.smooth = m_smooth.as<int8_t>();
-        check_hip(hipMemset(P_.smooth, 0, nc + nb), "hipMemset");
-        m_ftype.alloc(nc);
+        dzero(P_.smooth, nc + nb);
+        dalloc(m_ftype, nc);
         P_.ftype = m_ftype.as<int8_t>();
-        check_hip(hipMemset(P_.ftype, 0, nc), "hipMemset");
-        m_flat_prev.alloc(nc);
+        dzero(P_.ftype, nc);
+        dalloc(m_flat_prev, nc);
         P_.flat_prev = m_flat_prev.as<uint8_t>();
-        check_hip(hipMemset(P_.flat_prev, 0, nc), "hipMemset");
-        m_tlim.alloc(nc);
+        dzero(P_.flat_prev, nc);
+        dalloc(m_tlim, nc);
         P_.tlim = m_tlim.as<int8_t>();
-        check_hip(hipMemset(P_.tlim, 0, nc), "hipMemset");
+        dzero(P_.tlim, nc);
         P_.scnt = put(m_scnt, L_.scnt.data(), nc);
         P_.ccnt = put(m_ccnt, L_.ccnt.data(), nc);
         P_.sten = put(m_sten, L_.sten.data(), L_.sten.size());
@@ -583,10 +614,10 @@ class SolverImpl final : public DeviceSolver {
             P_.slot_d = reinterpret_cast<const v2*>(put(m_slot_d, d.data(), d.size()));
         }
         P_.slot_e = reinterpret_cast<const double2*>(put(m_slot_e, L_.slot_e.data(), 2 * nslot));
-        m_slot_dr.alloc(sizeof(R) * nslot);
+        dalloc(m_slot_dr, sizeof(R) * nslot);
         P_.slot_dr = m_slot_dr.as<R>();
-        check_hip(hipMemset(P_.slot_dr, 0, sizeof(R) * nslot), "hipMemset");
-        m_lsq.alloc(sizeof(v2) * nslot);
+        dzero(P_.slot_dr, sizeof(R) * nslot);
+        dalloc(m_lsq, sizeof(v2) * nslot);
         P_.lsq = m_lsq.as<v2>();
         {
             std::vector<R> rbf(L_.rbf.size());
@@ -609,11 +640,11 @@ class SolverImpl final : public DeviceSolver {
             for (size_t i = 0; i < g.size(); ++i) g[i] = (float)L_.fgeo[i];
             P_.fgeo32 = reinterpret_cast<const float4*>(put(m_fgeo32, g.data(), g.size()));
         }
-        m_fv.alloc(sizeof(v4) * 2 * nf);
+        dalloc(m_fv, sizeof(v4) * 2 * nf);
         P_.fv = m_fv.as<v4>();
-        m_fvT.alloc(sizeof(R) * 2 * nf);
+        dalloc(m_fvT, sizeof(R) * 2 * nf);
         P_.fvT = m_fvT.as<R>();
-        m_tTVD.alloc(sizeof(R) * 2 * nf);
+        dalloc(m_tTVD, sizeof(R) * 2 * nf);
         P_.tTVD = m_tTVD.as<R>();
         P_.ngp = L_.ngp;
         if (L_.ngp > 4) throw Error(3, "more than four flux quadrature points");
@@ -625,14 +656,14 @@ class SolverImpl final : public DeviceSolver {
             std::vector<R> gd(L_.fgd.size());
             for (size_t i = 0; i < gd.size(); ++i) gd[i] = (R)L_.fgd[i];
             P_.fgd = reinterpret_cast<const v4*>(put(m_fgd, gd.data(), gd.size()));
-            m_fvg.alloc(sizeof(v4) * 2 * nf * L_.ngp);
+            dalloc(m_fvg, sizeof(v4) * 2 * nf * L_.ngp);
             P_.fvg = m_fvg.as<v4>();
         }
-        m_flux.alloc(sizeof(double4) * (nf + nb));
+        dalloc(m_flux, sizeof(double4) * (nf + nb));
         P_.flux = m_flux.as<double4>();
         P_.flux32 = nullptr;
         if (sizeof(R) == 4 && L_.ngp == 0) {  // face-centre fluxes are floats: stored as such
-            m_flux32.alloc(sizeof(float4) * (nf ? nf : 1));
+            dalloc(m_flux32, sizeof(float4) * (nf ? nf : 1));
             P_.flux32 = m_flux32.as<float4>();
         }
         P_.bcell = put(m_bcell, L_.bcell.data(), nb);
@@ -658,11 +689,11 @@ class SolverImpl final : public DeviceSolver {
             for (int k = 0; k < ni; ++k) tc[4 * k + 1] = HUGE_VAL;
             P_.tctl = put(m_tctl, tc.data(), tc.size());
         }
-        m_res.alloc(sizeof(double) * (residual_blocks(L_.nc) + 1));
+        dalloc(m_res, sizeof(double) * (residual_blocks(L_.nc) + 1));
         P_.res_part = m_res.as<double>();
         const int dt_cap = std::max(kCflBlocks, 2 * flux_blocks(L_.nf, L_.nb));
         const size_t dt_bytes = (sizeof(double) * ni + 255) / 256 * 256;
-        m_dt.alloc(dt_bytes + 256 + sizeof(double) * dt_cap);
+        dalloc(m_dt, dt_bytes + 256 + sizeof(double) * dt_cap);
         P_.dt_cap = dt_cap;
         P_.dt = m_dt.as<double>();
         P_.dt_count = reinterpret_cast<unsigned int*>(m_dt.as<char>() + dt_bytes);
@@ -670,22 +701,30 @@ class SolverImpl final : public DeviceSolver {
         {
             const std::vector<double> init_dt(ni, 1e5);
             unsigned int zero = 0;
-            check_hip(hipMemcpy(P_.dt, init_dt.data(), sizeof(double) * ni, hipMemcpyHostToDevice), "dt init");
-            check_hip(hipMemcpy(P_.dt_count, &zero, 4, hipMemcpyHostToDevice), "dt init");
+            check_hip(hipMemcpyAsync(P_.dt, init_dt.data(), sizeof(double) * ni, hipMemcpyHostToDevice, stream_),
+                      "dt init");
+            check_hip(hipMemcpyAsync(P_.dt_count, &zero, 4, hipMemcpyHostToDevice, stream_), "dt init");
+            synchronize();
         }
         upload_state();
         // K3: least-squares weights, once (GPUevolve.cu:131-140)
         launch_lsq<R>(P_, stream_);
         check_hip(hipGetLastError(), "k_lsq");
         synchronize();
+        // the host keeps only what set_conserved needs (state, faces, boundary table)
+        release_host(ra_.RBFbasis, ra_.RBFgp, ra_.stencil, ra_.cellFaces, ra_.neighbour, ra_.GPD);
+        release_host(L_.rbf, L_.rbf_gp, L_.slot_d, L_.slot_e, L_.fgeo, L_.fgd);
+        release_host(L_.sten, L_.slot_face, L_.fcell, L_.halo);
     }
+
+    template <typename... V> static void release_host(V&... v) { ((void)std::decay_t<V>().swap(v), ...); }
 
     RefArrays ra_;
     BatchIndex bi_;
     DevLayout L_;
     DevPtrs<R> P_{};
+    StreamHolder sh_;  // before the buffers: destroyed after them
     hipStream_t stream_ = nullptr;
-    bool own_stream_ = false;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     hipEvent_t evq_[2] = {nullptr, nullptr};  // ends of the last two blocks of graph launches
     long long launches_ = 0, blocks_ = 0;

@@ -77,9 +77,9 @@ def request_stream(rate=20.0, duration=120.0, seed=3, nc_min=1e4, nc_max=2e6, t_
     return out
 
 
-def build_case(req, sc=None):
-    """Mesh and initial state of a request; precomputed on the host when
-    `sc` is given."""
+def build_case(req, sc=None, gpu=False):
+    """Mesh and initial state of a request; precomputed (the RBF part on the
+    GPU when gpu is set) when `sc` is given."""
     if req.kind == "corner":
         ny = max(4, int(round(math.sqrt(req.nc_target / 2.0))))
         nx = max(8, int(round(req.nc_target / ny)))
@@ -89,7 +89,7 @@ def build_case(req, sc=None):
         n = max(8, int(round(math.sqrt(req.nc_target / 1.5))))
         mesh, W0 = cases.riemann_case(n=n, seed=req.seed % 1000)
     if sc is not None:
-        mesh.precompute(sc)
+        mesh.precompute(sc, gpu=gpu)
     return mesh, W0
 
 
@@ -109,8 +109,10 @@ def footprint_bytes(mesh, precision):
 class Server:
     """Continuous-batching request loop (see the module docstring).
 
-    Host threads only prepare requests (mesh generation and precompute, no
-    GPU calls).  One scheduler thread makes every GPU call: it admits
+    Host threads prepare requests: mesh generation, precompute (its RBF part
+    on the GPU with gpu_precompute) and the solver with its state on the
+    device (the library's calls are safe from several threads).  One
+    scheduler thread steps the solvers: it admits
     prepared requests while their footprint fits in the HBM budget and a slot
     is free, enqueues chunks of graph-launched steps on each active solver's
     stream without waiting, and polls the streams; a request whose stream is
@@ -118,13 +120,16 @@ class Server:
     solver destroyed, memory and slot released)."""
 
     def __init__(self, precision="mixed", hbm_budget=64 << 30, max_active=8, chunk=50, prep_workers=2,
-                 log=None, poll_s=2e-4, gpu_precompute=False):
+                 log=None, poll_s=2e-4, gpu_precompute=False, max_prepared=None):
         self.precision = precision
         self.gpu_precompute = gpu_precompute
         self.budget = hbm_budget
         self.max_active = max_active
         self.chunk = chunk
         self.prep_workers = prep_workers
+        # prepared requests hold their mesh and coefficients in host memory
+        # until admitted: preparation waits while this many are ready
+        self.max_prepared = max_prepared if max_prepared is not None else max(2 * max_active, prep_workers)
         self.poll_s = poll_s
         self.log = log or (lambda m: None)
         self.sc = schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=precision)
@@ -134,15 +139,19 @@ class Server:
     def schemes_of(self, req):
         return self.sc if req.kind == "corner" else self.sc_riemann
 
-    # -- per-request steps, all on the scheduler thread
-    def _start(self, req, mesh, W0, t0):
+    def _create(self, req, mesh, W0):
+        """The request's solver, state on the device (preparation threads)."""
         g = BasicCUDASolver(mesh, self.schemes_of(req), W0)
-        req.times["start"] = time.perf_counter() - t0
+        g.mesh = None  # the solver holds its own copy: the host mesh can go
         req.cells = mesh.nc
+        return g
+
+    # -- per-request steps, all on the scheduler thread
+    def _start(self, req, g, t0):
+        req.times["start"] = time.perf_counter() - t0
         if req.kind == "riemann":
             g.set_time(0.0, req.t_end)
         self._enqueue(req, g)
-        return g
 
     def _enqueue(self, req, g):
         k = self.chunk if req.kind == "riemann" else min(self.chunk, req.max_steps - req.steps)
@@ -187,7 +196,8 @@ class Server:
         """One request alone, with the same chunking as in the loop."""
         if not mesh.precomputed:
             mesh.precompute(self.schemes_of(req), gpu=self.gpu_precompute)
-        g = self._start(req, mesh, W0, t0)
+        g = self._create(req, mesh, W0)
+        self._start(req, g, t0)
         while True:
             g.synchronize()
             if self._advance(req, g):
@@ -199,6 +209,7 @@ class Server:
         prep_q = queue.Queue()
         ready_q = queue.Queue()
         failed_q = queue.Queue()
+        slots = threading.Semaphore(self.max_prepared)  # prepared, not yet admitted
 
         def arrivals():
             for r in requests:
@@ -216,19 +227,21 @@ class Server:
                     r = prep_q.get()
                     if r is None:
                         return
+                    slots.acquire()
                     r.times["prep_start"] = time.perf_counter() - t0
                     try:
-                        # with the GPU precompute only the mesh is built here (no GPU
-                        # calls on this thread); the scheduler precomputes on admission
-                        mesh, W0 = build_case(r, None if self.gpu_precompute else self.schemes_of(r))
-                        need = None if self.gpu_precompute else footprint_bytes(mesh, self.precision)
+                        mesh, W0 = build_case(r, self.schemes_of(r), gpu=self.gpu_precompute)
+                        need = footprint_bytes(mesh, self.precision)
+                        g = self._create(r, mesh, W0)
+                        del mesh, W0
                     except Exception as e:  # a request that cannot be prepared fails alone
                         r.times["prepared"] = time.perf_counter() - t0
                         self._fail(r, e, t0)
+                        slots.release()
                         failed_q.put(r)
                         continue
                     r.times["prepared"] = time.perf_counter() - t0
-                    ready_q.put([r, mesh, W0, need])
+                    ready_q.put([r, g, need])
             finally:
                 ready_q.put(None)  # always posted, so the scheduler's producer count reaches zero
 
@@ -260,26 +273,16 @@ class Server:
                 else:
                     waiting.append(item)
             while waiting and len(active) < self.max_active:
-                if waiting[0][3] is None:  # precompute on the GPU, then size it
-                    try:
-                        waiting[0][1].precompute(self.schemes_of(waiting[0][0]), gpu=True)
-                        waiting[0][3] = footprint_bytes(waiting[0][1], self.precision)
-                    except Exception as e:
-                        r = waiting.pop(0)[0]
-                        r.times["admitted"] = time.perf_counter() - t0
-                        self._fail(r, e, t0)
-                        done.append(r)
-                        progressed = True
-                        continue
-                    waiting[0][0].times["precomputed"] = time.perf_counter() - t0
-                if used + waiting[0][3] > self.budget and active:
+                if used + waiting[0][2] > self.budget and active:
                     break
-                r, mesh, W0, need = waiting.pop(0)
+                r, g, need = waiting.pop(0)
                 r.times["admitted"] = time.perf_counter() - t0
                 progressed = True
+                slots.release()
                 try:
-                    g = self._start(r, mesh, W0, t0)
-                except Exception as e:  # e.g. the device is out of memory for this request
+                    self._start(r, g, t0)
+                except Exception as e:
+                    g.close()
                     self._fail(r, e, t0)
                     done.append(r)
                     continue
@@ -345,7 +348,7 @@ def summary_row(r):
 def fp64_reference(req, chunk=50):
     """The request solved alone on the all-FP64 path (the FP64 GPU path is
     pinned to the CPU oracle elsewhere), with the same stepping."""
-    srv = Server(precision="fp64", max_active=1, chunk=chunk)
+    srv = Server(precision="fp64", max_active=1, chunk=chunk, gpu_precompute=True)
     mesh, W0 = build_case(req)
     r = Request(req.rid, 0.0, req.kind, req.nc_target, req.seed, req.t_end, req.res_tol, req.max_steps)
     srv.solve(r, mesh, W0, time.perf_counter())
@@ -368,12 +371,13 @@ def main(argv=None):
     ap.add_argument("--sample", type=float, default=0.05, help="fraction checked against FP64")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--gpu-precompute", action="store_true",
-                    help="RBF precompute on the GPU at admission (on the scheduler thread; default: host threads)")
+                    help="RBF precompute on the GPU, from the preparation threads (default: on the host)")
+    ap.add_argument("--prep-workers", type=int, default=2, help="host threads preparing requests")
     args = ap.parse_args(argv)
     reqs = request_stream(args.rate, args.duration, args.seed, args.nc_min, args.nc_max, args.t_end, args.res_tol,
                           args.max_steps)
     srv = Server(max_active=args.max_active, chunk=args.chunk, log=lambda m: print(m, flush=True),
-                 gpu_precompute=args.gpu_precompute)
+                 gpu_precompute=args.gpu_precompute, prep_workers=args.prep_workers)
     done, wall = srv.run(reqs)
     ok = [r for r in done if not r.error]
     tts = np.array([r.times["finish"] - r.arrival for r in ok])
@@ -383,13 +387,27 @@ def main(argv=None):
     k = min(len(ok), max(1, int(np.ceil(args.sample * len(ok)))))
     order = sorted(ok, key=lambda r: r.rid)
     sample = [order[i] for i in sorted(rng.choice(len(order), size=k, replace=False))] if ok else []
-    errs = []
-    for r in sample:
+    import concurrent.futures
+
+    def err_of(r):
         ref = fp64_reference(r, args.chunk)
-        d = np.abs(r.result[:, 0] - ref.result[:, 0]).sum() / np.abs(ref.result[:, 0]).sum()
-        errs.append(float(d))
+        return float(np.abs(r.result[:, 0] - ref.result[:, 0]).sum() / np.abs(ref.result[:, 0]).sum())
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        errs = list(ex.map(err_of, sample))
+    def pct(vals):
+        vals = np.asarray([v for v in vals if v is not None], dtype=float)
+        return [round(float(np.percentile(vals, 50)), 3), round(float(np.percentile(vals, 99)), 3)] if len(vals) else None
+
+    rows = [summary_row(r) for r in ok]
+    corner_rel = [r.residual / r.residual0 for r in ok if r.kind == "corner" and r.residual0 > 0]
     print(json.dumps(dict(
         device=device_info()[0].strip(), requests=len(done), failed=len(done) - len(ok),
+        p50_p99_s=dict(prep=pct(x.get("prep_s") for x in rows), queue=pct(x.get("queue_s") for x in rows),
+                       gpu=pct(x.get("gpu_s") for x in rows)),
+        corner_residual_rel_median=float(np.median(corner_rel)) if corner_rel else None,
+        cells_total=int(sum(r.cells for r in ok)), gpu_precompute=args.gpu_precompute,
+        prep_workers=args.prep_workers, max_active=args.max_active, chunk=args.chunk,
         converged=dict(corner=sum(r.converged for r in ok if r.kind == "corner"),
                        corner_of=sum(r.kind == "corner" for r in ok),
                        riemann=sum(r.converged for r in ok if r.kind == "riemann"),

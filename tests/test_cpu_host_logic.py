@@ -61,7 +61,7 @@ def test_failed_preparation_does_not_stall_the_loop(afar, monkeypatch):
 
     from afar_cudaeuler2dmixo3rk3solver_amd import serve
 
-    def broken(req, sc=None):
+    def broken(req, sc=None, gpu=False):
         raise RuntimeError("stencil of request %d exceeds the limit" % req.rid)
 
     monkeypatch.setattr(serve, "build_case", broken)
