@@ -3,17 +3,29 @@ against FP32-mixed, errors against the exact solution and between the two.
 
     python -m afar_cudaeuler2dmixo3rk3solver_amd.precision_study [--sizes 64 128 256 512 1024] [--t-end 10]
 
-Case: doubly periodic [0, 10]^2, n x n quads (nodes jittered +-15%, seed 0)
+Case: doubly periodic [0, 10]^2, n x n quads (nodes jittered +-15% by default, --jitter; seed 0)
 split into 2 n^2 triangles (and, with --mesh quads or both, left unsplit); vortex of strength 5 on (rho, u, v, p) =
 (1, 1, 1, 1), gamma = 1.4 (R = 0.7142857, Cv = R / 0.4).  Initial state and
 errors are cell averages (5 x 5 conical product rule per triangle), so the
 measured order is not capped at 2 by point sampling.  At t = 10 the vortex has
-crossed the domain once and the exact solution equals the initial one.  Each
+crossed the domain once and the exact solution equals the initial one; at
+other times (--t-end) it is the initial vortex translated by (t, t).  Each
 run steps with the host-requested time step (CFL 0.5), the last step clipped
 to land on t_end, as the reference application does.
 
 Prints one JSON line per mesh and a final line with the least-squares slopes
 of log L1(rho) against log h (h = 10 / n).
+
+    python -m afar_cudaeuler2dmixo3rk3solver_amd.precision_study --attribution
+
+instead runs three solvers in lockstep (the first one's host time step for
+all): FP64 from W0, FP32-mixed from W0, and FP64 from W0 rounded to float
+(every later operation in FP64).  Every `every` steps it prints the relative
+L1 density difference of the second and third from the first and how many
+cells' final reconstruction types differ; at the end, the share of the mixed
+run's difference that sits in cells whose smoothness or type flag differed at
+any step.  Cases: the 200 x 100 Mach-2 corner (500 steps) and the vortex on
+128 x 128 triangles (2000 steps) and quads (900 steps, past t = 10).
 """
 import argparse
 import json
@@ -45,14 +57,18 @@ def errors(rho, ref):
     return dict(L1=float(d.mean()), Linf=float(d.max()), L1_rel=float(d.sum() / np.abs(ref).sum()))
 
 
-def study_one(n, t_end, split_p=1.0, seed=0, cOrder=7):
+def study_one(n, t_end, split_p=1.0, seed=0, cOrder=7, jitter=0.15):
+    """Errors at t_end against the exact cell averages at t_end (the initial
+    vortex translated by (t_end, t_end), periodically)."""
     R = cases.R_REF
     Cv = R / (GAMMA - 1.0)
     t0 = time.perf_counter()
-    mesh = cases.Mesh.periodic(n=n, L=10.0, jitter=0.15, seed=seed, split_p=split_p)
+    mesh = cases.Mesh.periodic(n=n, L=10.0, jitter=jitter, seed=seed, split_p=split_p)
     Ubar = cases.cell_average(mesh, lambda x, y: cases.vortex_conserved(x, y, gamma=GAMMA), n=5)
     W0 = cases.primitive_from_conserved(Ubar, R, Cv)
-    out = dict(mesh="triangles" if split_p == 1.0 else "quads", n=n, cells=mesh.nc, h=10.0 / n, t_end=t_end)
+    Uex = cases.cell_average(mesh, lambda x, y: cases.vortex_conserved(x, y, t=t_end, gamma=GAMMA), n=5)
+    out = dict(mesh="triangles" if split_p == 1.0 else "quads", n=n, cells=mesh.nc, h=10.0 / n, t_end=t_end,
+               jitter=jitter)
     res = {}
     for prec in ("fp64", "mixed"):
         sc = schemes(R=R, Cv=Cv, cOrder=cOrder, fOrder=2, maxCo=0.5, precision=prec)
@@ -66,9 +82,8 @@ def study_one(n, t_end, split_p=1.0, seed=0, cOrder=7):
         U = g.conserved()  # caller order
         g.close()
         res[prec] = U
-        e = errors(U[:, 0], Ubar[:, 0])
         out[prec] = dict(steps=steps, seconds=round(el, 3), Mcell_steps_per_s=round(mesh.nc * steps / el / 1e6, 1),
-                         rho_vs_exact=e)
+                         finite=bool(np.isfinite(U).all()), rho_vs_exact=errors(U[:, 0], Uex[:, 0]))
     d = np.abs(res["mixed"] - res["fp64"])
     out["mixed_vs_fp64"] = dict(
         rho_L1_rel=float(d[:, 0].sum() / np.abs(res["fp64"][:, 0]).sum()),
@@ -76,6 +91,57 @@ def study_one(n, t_end, split_p=1.0, seed=0, cOrder=7):
         rhoU_L1_rel=float(d[:, 1:3].sum() / np.abs(res["fp64"][:, 1:3]).sum()),
         rhoE_L1_rel=float(d[:, 3].sum() / np.abs(res["fp64"][:, 3]).sum()))
     return out
+
+
+def attribution_one(name, mesh, W0, nsteps, every, R=cases.R_REF, Cv=cases.CV_REF):
+    sc64 = schemes(R=R, Cv=Cv, cOrder=7, fOrder=2, maxCo=0.5)
+    scm = schemes(R=R, Cv=Cv, cOrder=7, fOrder=2, maxCo=0.5, precision="mixed")
+    mesh.precompute(sc64)
+    Wr = np.asarray(W0, dtype=np.float64).astype(np.float32).astype(np.float64)
+    a, b, c = BasicCUDASolver(mesh, sc64, W0), BasicCUDASolver(mesh, scm, W0), BasicCUDASolver(mesh, sc64, Wr)
+    nc, nb = a.nc, a.nb
+    ever = np.zeros(nc, dtype=bool)  # mesh order
+    t = 0.0
+
+    def flags(g):
+        return g.fetch("smooth", np.int8, nc + nb)[:nc], g.fetch("ftype", np.int8, nc)
+
+    for step in range(1, nsteps + 1):
+        dt = a.set_delta_t()
+        for g in (a, b, c):
+            g.evolve(dt)
+        t += dt
+        (sa, ta), (sb, tb) = flags(a), flags(b)
+        ever |= (sa != sb) | (ta != tb)
+        if step % every == 0 or step == nsteps:
+            Ua, Ub, Uc = a.conserved(), b.conserved(), c.conserved()
+            n = np.abs(Ua[:, 0]).sum()
+            print(json.dumps(dict(case=name, step=step, t=round(t, 5),
+                                  mixed_vs_fp64=float(np.abs(Ub[:, 0] - Ua[:, 0]).sum() / n),
+                                  fp64_from_rounded_W0_vs_fp64=float(np.abs(Uc[:, 0] - Ua[:, 0]).sum() / n),
+                                  type_diff_mixed=int((tb != ta).sum()),
+                                  type_diff_rounded_W0=int((flags(c)[1] != ta).sum()),
+                                  smooth_diff_mixed=int((sb != sa).sum()))), flush=True)
+    d = np.abs(Ub[:, 0] - Ua[:, 0])  # caller order
+    ev = np.zeros(nc, dtype=bool)
+    ev[mesh.permutation()] = ever
+    finite = bool(np.isfinite(d).all())
+    out = dict(case=name, cells=nc, steps=nsteps, cells_with_a_flag_difference=int(ever.sum()), finite=finite)
+    if finite and d.sum() > 0:
+        out["share_of_mixed_L1_in_those_cells"] = float(d[ev].sum() / d.sum())
+    print(json.dumps(out), flush=True)
+    for g in (a, b, c):
+        g.close()
+
+
+def attribution():
+    mesh, W0 = cases.corner_case()
+    attribution_one("corner 200x100", mesh, W0, 500, 50)
+    for split_p, fam, nsteps in ((1.0, "triangles", 2000), (0.0, "quads", 900)):
+        m = cases.Mesh.periodic(n=128, L=10.0, jitter=0.15, seed=0, split_p=split_p)
+        Ub = cases.cell_average(m, lambda x, y: cases.vortex_conserved(x, y, gamma=GAMMA), n=5)
+        R, Cv = cases.R_REF, cases.R_REF / (GAMMA - 1.0)
+        attribution_one("vortex %s 128" % fam, m, cases.primitive_from_conserved(Ub, R, Cv), nsteps, 100, R, Cv)
 
 
 def slope(hs, es):
@@ -88,12 +154,17 @@ def main(argv=None):
     ap.add_argument("--sizes", type=int, nargs="+", default=[64, 128, 256, 512, 1024])
     ap.add_argument("--t-end", type=float, default=10.0)
     ap.add_argument("--mesh", default="both", choices=["triangles", "quads", "both"])
+    ap.add_argument("--jitter", type=float, default=0.15, help="node jitter as a fraction of the spacing")
+    ap.add_argument("--attribution", action="store_true",
+                    help="mixed vs FP64 vs FP64 from the float-rounded state, in lockstep")
     args = ap.parse_args(argv)
+    if args.attribution:
+        return attribution()
     families = ["triangles", "quads"] if args.mesh == "both" else [args.mesh]
     for fam in families:
         rows = []
         for n in args.sizes:
-            r = study_one(n, args.t_end, split_p=1.0 if fam == "triangles" else 0.0)
+            r = study_one(n, args.t_end, split_p=1.0 if fam == "triangles" else 0.0, jitter=args.jitter)
             rows.append(r)
             print(json.dumps(r), flush=True)
         summarize(rows, fam, args)
@@ -102,7 +173,8 @@ def main(argv=None):
 def summarize(rows, fam, args):
     if len(rows) >= 2:
         hs = np.array([r["h"] for r in rows])
-        summary = dict(mesh=fam, device=device_info()[0].strip(), t_end=args.t_end, sizes=args.sizes)
+        summary = dict(mesh=fam, device=device_info()[0].strip(), t_end=args.t_end, sizes=args.sizes,
+                       jitter=args.jitter)
         for prec in ("fp64", "mixed"):
             summary["slope_L1_" + prec] = round(slope(hs, [r[prec]["rho_vs_exact"]["L1"] for r in rows]), 3)
             summary["slope_Linf_" + prec] = round(slope(hs, [r[prec]["rho_vs_exact"]["Linf"] for r in rows]), 3)

@@ -213,21 +213,23 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(mesh, sc, W0, budget_s=15.0, max_steps=100):
-    """The FP64 oracle (OpenMP) on the same mesh from the same state: steps
-    until budget_s.  Returns the baseline entry and the oracle (its initial
-    conserved state and the state after the timed steps, for the accuracy
-    check)."""
+def cpu_baseline(mesh, sc, W0, budget_s=15.0, check_steps=100):
+    """The FP64 oracle (OpenMP) on the same mesh from the same state: at
+    least check_steps steps, then on until budget_s.  Returns the baseline
+    entry and, for the accuracy check, the oracle's state after check_steps
+    steps."""
     from oracle import pyoracle as O
     ref = mesh.reference_arrays(sc, W0)
     orc = O.OracleState(ref, sc.R, sc.Cv, boundary_first_order=bool(sc.boundaryFirstOrder))
     del ref
     t0 = time.perf_counter()
-    n = 0
-    while n < max_steps:
+    n, check = 0, None
+    while True:
         orc.run(1, sc.maxCo)
         n += 1
-        if time.perf_counter() - t0 > budget_s:
+        if n == check_steps:
+            check = dict(U=orc.U.copy(), steps=n)
+        if n >= check_steps and time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
@@ -235,7 +237,7 @@ def cpu_baseline(mesh, sc, W0, budget_s=15.0, max_steps=100):
                  cpu=cpu_model(),
                  sample="%d RK3 steps (with CFL) of the same %d-cell mesh from the same state, FP64 oracle, "
                         "%d OpenMP threads, %.1f s" % (n, mesh.nc, cores, el))
-    return entry, dict(U=orc.U.copy(), steps=n)
+    return entry, check
 
 
 def accuracy_vs_oracle(mesh, sc, W0, check):
@@ -535,6 +537,25 @@ def run_batch(args):
     def physical(U):
         return bool(np.isfinite(U).all() and (U[:, 0] > 0).all())
     n_physical = sum(physical(b.conserved(k)) for _, b in bb.batches for k in range(len(b.meshes)))
+    bb.close()
+    # The roofline unit is one RK3 step of every mesh: the pipeline's
+    # compulsory bytes (no intermediate arrays) of each mesh, summed.
+    w = 8 if precision == "fp64" else 4
+    step_bytes = sum(survey_balg_per_cell_step(mesh_stats(m), w) * m.nc for m in meshes)
+    step_s = wall / args.steps
+    achieved = step_bytes / step_s / 1e9
+    roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBPS, unit="GB/s",
+                    frac=round(achieved / PEAK_HBM_GBPS, 4), traffic=None,
+                    unit_of_work="one RK3 step of all %d meshes (%d cells)" % (len(meshes), cells),
+                    alg_bytes_per_step=int(step_bytes), alg_bytes_per_cell_step=round(step_bytes / cells, 1),
+                    avg_step_ms=round(step_s * 1e3, 5),
+                    traffic_note="no counter summary for the batch: HBM bytes per step not measured")
+    cpu = None
+    if not args.no_cpu:
+        # the oracle on the median-sized mesh of the batch, from its starting state
+        k = int(np.argsort([m.nc for m in meshes])[len(meshes) // 2])
+        cpu, _ = cpu_baseline(meshes[k], A.schemes(**dict(sc_kwargs, precision="fp64")), W0s[k])
+        cpu["sample"] = "mesh %d of the batch (median size): " % k + cpu["sample"]
     out = {
         "metric": "Mcell-RK3-steps/s", "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
@@ -547,9 +568,8 @@ def run_batch(args):
                    "buckets": {str(k): v for k, v in bb.tuning.items()}, "boundary_first_order": True,
                    "meshes_finite_positive_after_timed_steps": n_physical,
                    "timing": "host wall clock around the enqueue of all buckets' steps and their completion"},
-        "roofline": None, "cpu_baseline": None,
+        "roofline": roofline, "cpu_baseline": cpu,
     }
-    bb.close()
     out["command"] = " ".join(["python", "bench.py"] + sys.argv[1:])
     print(json.dumps(out))
 
