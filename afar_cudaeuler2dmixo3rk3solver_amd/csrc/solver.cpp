@@ -124,9 +124,11 @@ class SolverImpl final : public DeviceSolver {
         upload();
     }
     ~SolverImpl() override {
+        // no launch of the graph is in flight when it is destroyed; the
+        // buffers' frees follow, in stream order
+        if (stream_) (void)hipStreamSynchronize(stream_);
         if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
-        if (stream_) (void)hipStreamSynchronize(stream_);  // the buffers' frees follow, in stream order
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
         for (hipEvent_t e : evq_)
@@ -324,7 +326,9 @@ class SolverImpl final : public DeviceSolver {
         } cp[2];
         struct Cleanup {
             Copy* c;
+            hipStream_t s;
             ~Cleanup() {
+                (void)hipStreamSynchronize(s);  // an error may leave launches of the copies in flight
                 std::lock_guard<std::mutex> lk(runtime_mutex());
                 for (int b = 0; b < 2; ++b) {
                     if (c[b].ge) (void)hipGraphExecDestroy(c[b].ge);
@@ -333,7 +337,7 @@ class SolverImpl final : public DeviceSolver {
                         if (e) (void)hipEventDestroy(e);
                 }
             }
-        } cleanup{cp};
+        } cleanup{cp, stream_};
         synchronize();
         std::unique_lock<std::mutex> build_lock(runtime_mutex());
         for (int b = 0; b < 2; ++b) {

@@ -26,11 +26,14 @@ struct SolverConfig {
 };
 
 // Held by every library call that allocates or frees device memory, creates
-// or destroys a solver, or captures or destroys a graph.  HIP frees
-// synchronise the device, and a stream under capture must not be waited on:
-// serialising these calls makes solvers safe to create, step and destroy from
-// different host threads.  Kernel and graph launches, copies and queries on a
-// solver's own stream do not take it.
+// or destroys a solver, or captures or destroys a graph.  A stream under
+// capture must not be waited on, and some of these calls wait on the device
+// (a plain hipFree, the synchronous copies of an upload): serialising them
+// with every capture makes solvers safe to create, step and destroy from
+// different host threads.  Solver buffers are stream-ordered (hipMallocAsync /
+// hipFreeAsync on the solver's stream), so a destroy waits for its own stream
+// only.  Kernel and graph launches, copies and queries on a solver's own
+// stream do not take the lock.
 std::mutex& runtime_mutex();
 
 // kernel kinds reported by profile()

@@ -23,8 +23,8 @@
  * stepped and destroyed on different host threads at the same time.  The
  * calls that allocate or free device memory, create or destroy a solver,
  * batch or device, or capture a step graph take one process-wide lock, so
- * they never overlap each other (a HIP free waits for the whole device, and
- * a stream being captured must not be waited on); kernel launches, copies
+ * they never overlap each other (a stream being captured must not be waited
+ * on, and some of these calls wait on the device); kernel launches, copies
  * and queries on a handle's own stream do not take it.  One handle must not
  * be used by two threads at once.  Each solver runs on one HIP stream: its
  * own (afar_solver_create), or the caller's (afar_solver_create_on_stream,
