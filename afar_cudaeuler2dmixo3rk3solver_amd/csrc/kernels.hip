@@ -877,11 +877,26 @@ __device__ __forceinline__ void roe_flux(R& rhoFlux, R& rhoUxFlux, R& rhoUyFlux,
                           l3 * r3 * (hT + cT * VT_)) * magSf;
 }
 
+// Waves per SIMD the register allocator must allow for the face-centre flux
+// pass, per working type (build-time tuning knobs; 1 = no constraint).  The
+// FP64 pass takes 98-100 VGPRs (5 waves) unconstrained; 6 and 8 waves spill
+// (36 and 132 bytes per lane) and measured 26% and 108% slower per launch.
+#ifndef AFAR_FLUX_WPE_D
+#define AFAR_FLUX_WPE_D 1
+#endif
+#ifndef AFAR_FLUX_WPE_F
+#define AFAR_FLUX_WPE_F 1
+#endif
+template <typename R, bool GP> struct FluxWaves {
+    static constexpr int value = GP ? 1 : (sizeof(R) == 8 ? AFAR_FLUX_WPE_D : AFAR_FLUX_WPE_F);
+};
+
 // DT: also reduce the CFL bound of every internal face to one partial minimum
 // per block (dt_part[blockIdx.x]); the bound is a function of the
 // primitive state alone, which no kernel of the stage has changed yet.
 template <typename R, bool DT, bool GP>
-__global__ __launch_bounds__(256) void k_flux(DevPtrs<R> p, double Rgas, double Cv, double CFL) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R, GP>::value, 8))) void k_flux(
+    DevPtrs<R> p, double Rgas, double Cv, double CFL) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
