@@ -53,8 +53,13 @@ struct DevPtrs {
     const double4* fgeo;       // 2*nf: (Sfx, Sfy, magSf, deltaR), (dxo, dyo, dxn, dyn)
     const float4* fgeo32;      // FP32-mixed: fgeo rounded to float (the flux pass without the CFL bound
                                // uses the geometry in float only); else null
-    typename VT<R>::v4* fv;    // 2*nf reconstructed face values, [2f + side]
-    R* fvT;                    // 2*nf reconstructed face temperatures (the part the BVD test reads)
+    // 2*nf reconstructed face values [2f + side].  FP64: one array per
+    // variable (fvp, fvu, fvv, fvT; fv null).  FP32-mixed: (p, Ux, Uy, T)
+    // records in fv and the temperatures again in fvT (fvp, fvu, fvv null):
+    // 4-byte scattered stores into four arrays measured slower than one
+    // 16-byte record.  The BVD test reads fvT alone.
+    typename VT<R>::v4* fv;
+    R *fvp, *fvu, *fvv, *fvT;
     R* tTVD;                   // 2*nf the BVD test's limited linear face temperature of that side's cell,
                                // Tc + BJ(raw T limiter) * (grad T . d), formed by the reconstruction
     int8_t* tlim;              // nc: BJ(raw T limiter) < VALID_LIM_FACTOR (the BVD test's limiter condition)

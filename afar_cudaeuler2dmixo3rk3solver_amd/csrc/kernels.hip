@@ -165,6 +165,20 @@ __device__ __forceinline__ void stage_stencil(uint16_t* sSt, const uint16_t* ste
     for (int i = t; i < n16; i += nthreads) dst[i] = src[i];
 }
 
+// The reconstructed state of face side i (= 2f + side): FP64 one array per
+// variable, FP32 a record plus the temperature again (the layouts of DevPtrs).
+template <typename R>
+__device__ __forceinline__ void store_face_state(const DevPtrs<R>& p, int i, const typename VT<R>::v4& o) {
+    if constexpr (sizeof(R) == 8) {
+        p.fvp[i] = o.x;
+        p.fvu[i] = o.y;
+        p.fvv[i] = o.z;
+    } else {
+        p.fv[i] = o;
+    }
+    p.fvT[i] = o.w;
+}
+
 template <typename R>
 __device__ __forceinline__ void limiter_process(R& lim, R& phif, R phic, R phiMax, R phiMin, R gx, R gy, R dx, R dy) {
     // cudaLimiter.h:35-50
@@ -260,8 +274,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
             o.y = wc.y + guxx * dx + guyx * dy;
             o.z = wc.z + guxy * dx + guyy * dy;
             o.w = wc.w + gTx * dx + gTy * dy;
-            p.fv[2 * f + isNei] = o;
-            p.fvT[2 * f + isNei] = o.w;
+            store_face_state(p, 2 * f + isNei, o);
             p.tTVD[2 * f + isNei] = wc.w + lT * (gTx * dx + gTy * dy);
             if constexpr (GP)
                 for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
@@ -319,8 +332,7 @@ __global__ __launch_bounds__(TILE) void k_reconstruct(DevPtrs<R> p) {
         limiter_process(limT, phif, wc.w, TMax, TMin, gTx, gTy, dx, dy);
         o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
         isNotTrouble = phif > (R)0;
-        p.fv[2 * f + isNei] = o;
-        p.fvT[2 * f + isNei] = o.w;
+        store_face_state(p, 2 * f + isNei, o);
         if constexpr (GP)
         for (int g = 0; g < p.ngp; ++g) {  // flux points (extension), as in k_reconstruct_q
             const R* ag = p.rbf_gp + sell(c, (g * maxC + fi) * (maxS + 1), p.ngp * KR);
@@ -543,8 +555,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
             o.y = wc.y + guxx * dx + guyx * dy;
             o.z = wc.z + guxy * dx + guyy * dy;
             o.w = wc.w + gTx * dx + gTy * dy;
-            p.fv[2 * f + isNei] = o;
-            p.fvT[2 * f + isNei] = o.w;
+            store_face_state(p, 2 * f + isNei, o);
             p.tTVD[2 * f + isNei] = wc.w + lT * (gTx * dx + gTy * dy);
             if constexpr (GP)
                 for (int g = 0; g < p.ngp; ++g) {  // flux points (extension): the same linear state there
@@ -606,8 +617,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
         limiter_process(limT, phif, wc.w, TMax, TMin, gTx, gTy, dx, dy);
         o.w = (R)(rT >= K<R>::SMALL) * Tf + (R)(rT < K<R>::SMALL) * phif;
         faceNotTrouble = phif > (R)0;
-        p.fv[2 * f + isNei] = o;
-        p.fvT[2 * f + isNei] = o.w;
+        store_face_state(p, 2 * f + isNei, o);
         // flux points (extension): RBF sums with the point's coefficients,
         // or the unlimited linear state where the compact range is flat
         if constexpr (GP)
@@ -985,7 +995,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R
         };
         if constexpr (!GP) {
             R rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux;
-            point_flux(dxL, dyL, dxR, dyR, p.fv[2 * f], p.fv[2 * f + 1], rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
+            v4 fL, fR;
+            if constexpr (sizeof(R) == 8) {  // both sides' states: one 2-vector per variable
+                const v2 sp = reinterpret_cast<const v2*>(p.fvp)[f], su = reinterpret_cast<const v2*>(p.fvu)[f];
+                const v2 sv = reinterpret_cast<const v2*>(p.fvv)[f], sT = reinterpret_cast<const v2*>(p.fvT)[f];
+                fL = v4{sp.x, su.x, sv.x, sT.x};
+                fR = v4{sp.y, su.y, sv.y, sT.y};
+            } else {
+                fL = p.fv[2 * f];
+                fR = p.fv[2 * f + 1];
+            }
+            point_flux(dxL, dyL, dxR, dyR, fL, fR, rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
             if constexpr (sizeof(R) == 4)
                 p.flux32[f] = float4{rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux};
             else

@@ -438,13 +438,27 @@ class SolverImpl final : public DeviceSolver {
         pick("lim", P_.lim, sizeof(v4) * nc);
         pick("smooth", P_.smooth, nc + nb);
         pick("ftype", P_.ftype, nc);
-        pick("fv", P_.fv, sizeof(v4) * 2 * nf);
+        pick("fv", P_.fv ? (const void*)P_.fv : (const void*)P_.fvp, sizeof(v4) * 2 * nf);
         pick("fvg", P_.fvg, sizeof(v4) * 2 * nf * L_.ngp);
         pick("flux", P_.flux, sizeof(double4) * (nf + nb));
         pick("lsq", P_.lsq, sizeof(v2) * (size_t)L_.nwaves * L_.maxC * 64);
         if (!src) throw Error(1, std::string("unknown device array ") + what);
         if (bytes != have) throw Error(1, std::string("size mismatch fetching ") + what);
         synchronize();
+        if (strcmp(what, "fv") == 0 && !P_.fv) {  // (p, Ux, Uy, T) records from the per-variable arrays
+            std::vector<R> a[4];
+            const R* srcs[4] = {P_.fvp, P_.fvu, P_.fvv, P_.fvT};
+            for (int k = 0; k < 4; ++k) {
+                a[k].resize(2 * nf);
+                if (nf)
+                    check_hip(hipMemcpyAsync(a[k].data(), srcs[k], sizeof(R) * 2 * nf, hipMemcpyDeviceToHost, stream_),
+                              "debug copy");
+            }
+            synchronize();
+            v4* o = static_cast<v4*>(out);
+            for (size_t i = 0; i < 2 * nf; ++i) o[i] = v4{a[0][i], a[1][i], a[2][i], a[3][i]};
+            return;
+        }
         if (P_.flux32 && strcmp(what, "flux") == 0) {  // internal faces from their float store
             std::vector<float4> q(nf);
             if (nf)
@@ -644,8 +658,19 @@ class SolverImpl final : public DeviceSolver {
             for (size_t i = 0; i < g.size(); ++i) g[i] = (float)L_.fgeo[i];
             P_.fgeo32 = reinterpret_cast<const float4*>(put(m_fgeo32, g.data(), g.size()));
         }
-        dalloc(m_fv, sizeof(v4) * 2 * nf);
-        P_.fv = m_fv.as<v4>();
+        P_.fv = nullptr;
+        P_.fvp = P_.fvu = P_.fvv = nullptr;
+        if (sizeof(R) == 8) {  // one array per variable
+            dalloc(m_fvp, sizeof(R) * 2 * nf);
+            P_.fvp = m_fvp.as<R>();
+            dalloc(m_fvu, sizeof(R) * 2 * nf);
+            P_.fvu = m_fvu.as<R>();
+            dalloc(m_fvv, sizeof(R) * 2 * nf);
+            P_.fvv = m_fvv.as<R>();
+        } else {  // records, the temperatures again in fvT
+            dalloc(m_fv, sizeof(v4) * 2 * nf);
+            P_.fv = m_fv.as<v4>();
+        }
         dalloc(m_fvT, sizeof(R) * 2 * nf);
         P_.fvT = m_fvT.as<R>();
         dalloc(m_tTVD, sizeof(R) * 2 * nf);
@@ -738,7 +763,7 @@ class SolverImpl final : public DeviceSolver {
     int lanes_ = 1;  // threads per cell of the reconstruction kernel
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_slot_dr, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fgeo32,
-        m_fv, m_flux, m_flux32, m_bcell, m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res,
+        m_fv, m_fvp, m_fvu, m_fvv, m_flux, m_flux32, m_bcell, m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res,
         m_rbf_gp, m_fgd, m_fvg, m_flat_prev, m_tlim, m_tTVD;
 };
 

@@ -71,10 +71,11 @@ def kernel_bytes(st, w=8, ngp=0):
     nc, nf, nb, sS, sC = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"]
     sR, sfull = st["sum_int_coef_full"], st["int_slots_full"]
     ns = 2 * nf  # internal face slots
-    # writes: gradients 8w, limiters 4w, smoothness flag and limiter condition per cell; O3 face state 4w,
-    # its temperature w and the BVD test's limited linear temperature w per internal face side
+    # writes: gradients 8w, limiters 4w, smoothness flag and limiter condition per cell; O3 face state 4w
+    # (FP64: one array per variable, the temperatures being what the BVD test reads; FP32: records and the
+    # temperatures again, w) and the BVD test's limited linear temperature w per internal face side
     recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * sfull + 2 * w * sC + w * sR
-             + 8 * w * nc + 4 * w * nc + 2 * nc + 4 * w * ns + w * ns + w * ns)
+             + 8 * w * nc + 4 * w * nc + 2 * nc + 4 * w * ns + w * ns + (w * ns if w == 4 else 0))
     # bvd reads flags, counts, stencil ids, face slots, both sides' O3 face temperatures and this side's
     # limited linear one, the limiter condition; writes the types (the face pass applies the limiter transform)
     bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + w * ns + w * ns + nc + nc)
