@@ -196,9 +196,6 @@ template <typename R> __device__ __forceinline__ R to_bj(R y) { return fmin((R)2
 template <typename R> __device__ __forceinline__ R to_venkat(R y) {
     return fmin((y * y + y + K<R>::VSMALL) / (y * y + (R)0.5 + (R)0.5 * y + K<R>::VSMALL), (R)1);
 }
-template <typename R> __device__ __forceinline__ R transform_limiter(R lim, int isShock) {
-    return (R)isShock * to_venkat(lim) + (R)(1 - isShock) * to_bj(lim);
-}
 
 // GP: the flux-quadrature extension's point states (a separate instantiation,
 // so the reference path carries none of its registers)
@@ -799,9 +796,18 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
 }
 
 // The limiter the face pass uses: the transform of K6 applied to the raw K5
-// limiter with the cell's final type, or zero for a first-order cell.
+// limiter with the cell's final type, or zero for a first-order cell.  The
+// reference blends both transforms, isShock * venkat + (1 - isShock) * bj
+// (cudaLimiter.h:453-461); only the selected one is evaluated here, so that a
+// wavefront without shock cells skips the Venkatakrishnan divisions.  The
+// value is the blend's bit for bit: both transforms of a raw limiter >= 0 are
+// finite (fmin(., 1) of a finite or NaN value), the Venkatakrishnan value is
+// positive, and adding zero turns a BJ value of -0, where the blend gives
+// +0 + -0 = +0, into +0.
 template <typename R> __device__ __forceinline__ R face_limiter(R raw, int isShock) {
-    return raw < (R)0 ? (R)0 : transform_limiter(raw, isShock);
+    if (raw < (R)0) return (R)0;
+    const R t = isShock ? to_venkat(raw) : to_bj(raw);
+    return t + (R)0;
 }
 
 // ---------------------------------------------------------------- K7 flux
