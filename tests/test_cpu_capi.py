@@ -48,6 +48,9 @@ def test_bad_polygon_mesh_rejected(afar):
     # an edge with no patch
     with pytest.raises(afar.AfarError, match="boundary edge without a patch"):
         afar.Mesh.from_polygons([[0, 0], [1, 0], [1, 1], [0, 1]], [0, 4], [0, 1, 2, 3], [[0, 1]], [0], [4])
+    # no cells at all
+    with pytest.raises(afar.AfarError, match="empty mesh"):
+        afar.Mesh.from_polygons(np.zeros((0, 2)), [0], np.zeros(0, int), np.zeros((0, 2), int), np.zeros(0, int), [4])
 
 
 def test_polygon_mesh_equals_generator(afar):

@@ -335,3 +335,57 @@ def test_residual_norm_matches_oracle(afar):
     say("  residual %.6e (gpu) %.6e (oracle)" % (r_gpu, r_orc))
     assert abs(r_gpu - r_orc) <= 1e-12 * r_orc
     assert np.array_equal(gpu.fetch("ftype", np.int8, mesh.nc), orc.final_types)
+
+
+def tiny_case(afar, kind):
+    """The smallest meshes: one unit square whose four faces are all
+    boundary faces (no internal face: the CFL bound has none to take, so the
+    time step is the reference's 1e5 cap, cudaFieldOperation.h:124-180), two
+    squares sharing one face, one triangle, a strip of three quads."""
+    if kind == "one square":
+        pts = [[0, 0], [1, 0], [1, 1], [0, 1]]
+        ptr, cells, be = [0, 4], [0, 1, 2, 3], [[0, 1], [1, 2], [2, 3], [3, 0]]
+        bp, labels = [0, 0, 0, 0], [4]
+    elif kind == "two squares":
+        pts = [[0, 0], [1, 0], [2, 0], [2, 1], [1, 1], [0, 1]]
+        ptr, cells = [0, 4, 8], [0, 1, 4, 5, 1, 2, 3, 4]
+        be, bp, labels = [[0, 1], [1, 2], [2, 3], [3, 4], [4, 5], [5, 0]], [0, 0, 1, 0, 0, 1], [2, 4]
+    elif kind == "one triangle":
+        pts = [[0, 0], [1, 0.2], [0.3, 1]]
+        ptr, cells, be, bp, labels = [0, 3], [0, 1, 2], [[0, 1], [1, 2], [2, 0]], [0, 0, 0], [0]
+    else:  # strip of three quads
+        pts = [[0, 0], [1, 0], [2, 0.1], [3, 0], [3, 1], [2, 1.1], [1, 1], [0, 1]]
+        ptr, cells = [0, 4, 8, 12], [0, 1, 6, 7, 1, 2, 5, 6, 2, 3, 4, 5]
+        be = [[0, 1], [1, 2], [2, 3], [3, 4], [4, 5], [5, 6], [6, 7], [7, 0]]
+        bp, labels = [0, 0, 0, 1, 0, 0, 0, 2], [2, 4, 1]
+    mesh = afar.Mesh.from_polygons(np.array(pts, float), np.array(ptr), np.array(cells), np.array(be),
+                                   np.array(bp), labels)
+    nc = mesh.nc
+    k = np.arange(nc)
+    W0 = np.stack([1.0 + 0.2 * k, 0.3 - 0.1 * k, 0.05 * k, 1.4 + 0.1 * (k % 2)], 1)
+    return mesh, W0
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("kind", ["one square", "two squares", "one triangle", "strip of three quads"])
+def test_tiny_meshes_parity(afar, kind, lanes):
+    mesh, W0 = tiny_case(afar, kind)
+    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, reconLanes=lanes)
+    mesh.precompute(sc)
+    gpu, orc, ref = make_pair(afar, mesh, W0, sc)
+    say("case: %s (nc=%d, nf=%d, nb=%d), lanes %d, 20 steps" % (kind, mesh.nc, mesh.sizes()["nf"],
+                                                                 mesh.sizes()["nb"], lanes))
+    for s in range(20):
+        dt_o = orc.adjust_time_step(sc.maxCo)
+        dt_g = gpu.set_delta_t()
+        assert dt_o == dt_g, (s, dt_o, dt_g)
+        orc.evolve(dt_o)
+        gpu.evolve(dt_g)
+    if mesh.sizes()["nf"] == 0:
+        assert dt_g == 1e5
+    U = gpu.conserved()[mesh.permutation()]
+    assert np.all(np.isfinite(U))
+    assert np.array_equal(U, orc.U)
+    g = afar.BasicCUDASolver(mesh, sc, W0)  # device time steps: the same trajectory
+    g.evolve_auto(20)
+    assert np.array_equal(g.conserved()[mesh.permutation()], orc.U)
