@@ -1202,8 +1202,14 @@ __global__ __launch_bounds__(256) void k_update(DevPtrs<R> p, double Rgas, doubl
     const double rhoUy = b1 * uo.z + b2 * u.z + b3 * dt * (-r2 / V);
     const double rhoE = b1 * uo.w + b2 * u.w + b3 * dt * (-r3 / V);
     const double4 un = double4{rho, rhoUx, rhoUy, rhoE};
-    p.U[c] = un;
-    if (last) p.Uold[c] = un;
+    // The last stage's result is the next step's starting state, which the
+    // next first stage reads from Uold alone; U would only be overwritten by
+    // that stage, so it is not written here (the host reads the state from
+    // Uold after a full step).
+    if (last)
+        p.Uold[c] = un;
+    else
+        p.U[c] = un;
     const double pr = (gamma - 1.0) * (rhoE - 0.5 * (rhoUx * rhoUx + rhoUy * rhoUy) / rho);
     const double4 w = double4{pr, rhoUx / rho, rhoUy / rho, pr / (Rgas * rho)};
     typename VT<R>::v4 wr;

@@ -148,6 +148,7 @@ class SolverImpl final : public DeviceSolver {
     void evolve(double dt) override {
         if (ninst() > 1) throw Error(1, "a batch steps with the device time step only");
         enqueue_stages(dt, nullptr);
+        state_in_uold_ = true;
         check_hip(hipGetLastError(), "evolve");
     }
 
@@ -202,13 +203,13 @@ class SolverImpl final : public DeviceSolver {
     }
 
     void get_conserved(double* U) override {
-        check_hip(hipMemcpyAsync(U, P_.U, sizeof(double4) * L_.nc, hipMemcpyDeviceToHost, stream_), "U copy");
+        check_hip(hipMemcpyAsync(U, state_U(), sizeof(double4) * L_.nc, hipMemcpyDeviceToHost, stream_), "U copy");
         synchronize();
     }
 
     void get_conserved_range(int c0, int n, double* U) override {
         if (c0 < 0 || n < 0 || c0 + n > L_.nc) throw Error(1, "cell range out of bounds");
-        check_hip(hipMemcpyAsync(U, P_.U + c0, sizeof(double4) * n, hipMemcpyDeviceToHost, stream_), "U copy");
+        check_hip(hipMemcpyAsync(U, state_U() + c0, sizeof(double4) * n, hipMemcpyDeviceToHost, stream_), "U copy");
         synchronize();
     }
 
@@ -295,6 +296,7 @@ class SolverImpl final : public DeviceSolver {
             for (int i = 0; i < nsteps; ++i) launch_graph();
         } else {
             for (int i = 0; i < nsteps; ++i) enqueue_stages(0.0, P_.dt);
+            if (nsteps > 0) state_in_uold_ = true;
         }
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         check_hip(hipEventSynchronize(ev1_), "hipEventSynchronize");
@@ -362,6 +364,7 @@ class SolverImpl final : public DeviceSolver {
         for (int i = 0; i < nsteps; ++i) {
             if (i >= 2) collect(cp[i & 1]);  // step i-2 used this copy
             check_hip(hipGraphLaunch(cp[i & 1].ge, stream_), "hipGraphLaunch");
+            state_in_uold_ = true;
         }
         check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord");
         for (int i = std::max(0, nsteps - 2); i < nsteps; ++i) collect(cp[i & 1]);
@@ -395,6 +398,7 @@ class SolverImpl final : public DeviceSolver {
             check_hip(hipEventElapsedTime(&t, ev0_, ev1_), "hipEventElapsedTime");
             ms[k] = t / reps;
         }
+        state_in_uold_ = false;  // the update launches above wrote U
         // launches per RK3 step with the device-side time step
         count[kKernCfl] = 0;
         count[kKernRecon] = count[kKernBvd] = count[kKernUpdate] = 3;
@@ -430,7 +434,7 @@ class SolverImpl final : public DeviceSolver {
         };
         const size_t nc = L_.nc, nf = L_.nf, nb = L_.nb;
         pick("W", P_.W, sizeof(v4) * (nc + nb));
-        pick("U", P_.U, sizeof(double4) * nc);
+        pick("U", state_U(), sizeof(double4) * nc);
         pick("gp", P_.gp, sizeof(v2) * nc);
         pick("gu", P_.gu, sizeof(v2) * nc);
         pick("gv", P_.gv, sizeof(v2) * nc);
@@ -481,7 +485,10 @@ class SolverImpl final : public DeviceSolver {
         launch_reconstruct<R>(P_, L_.tile, lds_recon_, stream_, lanes_);
         if (upto > 0) launch_bvd<R>(P_, L_.tile, lds_bvd_, stream_);
         if (upto > 1) launch_flux<R>(P_, cfg_.R, cfg_.Cv, stream_);
-        if (upto > 2) launch_update<R>(P_, cfg_.R, gamma(), dt, nullptr, b1(stage), b2(stage), b3(stage), stage == 2, stream_);
+        if (upto > 2) {
+            launch_update<R>(P_, cfg_.R, gamma(), dt, nullptr, b1(stage), b2(stage), b3(stage), stage == 2, stream_);
+            state_in_uold_ = stage == 2;
+        }
         check_hip(hipGetLastError(), "stage kernels");
     }
 
@@ -539,6 +546,7 @@ class SolverImpl final : public DeviceSolver {
     static constexpr int kGraphBlock = 256;
     void launch_graph() {
         check_hip(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
+        state_in_uold_ = true;
         if (++launches_ % kGraphBlock) return;
         hipEvent_t& e = evq_[blocks_ & 1];
         if (blocks_ >= 2) check_hip(hipEventSynchronize(e), "hipEventSynchronize");
@@ -566,6 +574,7 @@ class SolverImpl final : public DeviceSolver {
     }
 
     void upload_state() {
+        state_in_uold_ = false;  // U and Uold both receive the state
         const int nc = L_.nc, nb = L_.nb;
         std::vector<v4> w(nc + nb);
         for (int i = 0; i < nc + nb; ++i) {
@@ -759,6 +768,11 @@ class SolverImpl final : public DeviceSolver {
     long long launches_ = 0, blocks_ = 0;
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t graph_exec_ = nullptr;
+    // Where the conserved state is: a full step's last stage writes it to
+    // Uold only (the next step starts from there); a first or second stage
+    // writes U.  Host reads take it from state_U().
+    bool state_in_uold_ = false;
+    double4* state_U() const { return state_in_uold_ ? P_.Uold : P_.U; }
     size_t lds_recon_ = 0, lds_bvd_ = 0, lds_m_off_ = 0, lds_bvd_sten_off_ = 0;
     int lanes_ = 1;  // threads per cell of the reconstruction kernel
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,

@@ -84,8 +84,9 @@ def kernel_bytes(st, w=8, ngp=0):
     fw = 4 * w if (w == 4 and not ngp) else 32
     geo = 8 * w
     flux = 8 * nf + geo * nf + 8 * w * nf + nc * (4 * w + 4 * w + 8 * w + 1) + 32 * nb + 4 * w * nb + fw * nf + 32 * nb
-    # U and Uold are read in the second and third stages, Uold alone in the first
-    update = nc + 4 * sC + fw * nf + 32 * nb + 32 * nc * 5 / 3 + 8 * nc + 65 * nb + 32 * nc + 4 * w * (nc + nb) + 32 * nc / 3
+    # U and Uold are read in the second and third stages, Uold alone in the first; the first two stages
+    # write U, the last writes Uold only
+    update = nc + 4 * sC + fw * nf + 32 * nb + 32 * nc * 5 / 3 + 8 * nc + 65 * nb + 32 * nc + 4 * w * (nc + nb)
     cfl = 8 * nf + 48 * nf + 4 * w * nc
     if ngp:
         recon += ngp * (w * sR + 4 * w * ns + 4 * w * ns)
