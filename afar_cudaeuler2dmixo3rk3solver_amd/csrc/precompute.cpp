@@ -5,6 +5,7 @@
 // Follows src/CUDA/cudaEuler2DmixO3RK3/basicCUDAsolver.cpp:261-694 and
 // src/gaussQuadrature/d2GaussQuadrature.C:23-547.
 #include <math.h>
+#include <sched.h>
 #include <stdio.h>
 
 #include <algorithm>
@@ -19,10 +20,25 @@
 
 namespace afar {
 
+// Threads of a parallel host loop: AFAR_HOST_THREADS if set, else the CPUs
+// this process may run on (its affinity mask, which a container's CPU share
+// narrows; hardware_concurrency counts the whole machine), capped by
+// OMP_NUM_THREADS when that is set.
+static unsigned host_threads() {
+    if (const char* e = getenv("AFAR_HOST_THREADS")) return (unsigned)std::max(1, atoi(e));
+    unsigned hw = std::thread::hardware_concurrency();
+    cpu_set_t set;
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) hw = std::min(hw, (unsigned)std::max(1, CPU_COUNT(&set)));
+    if (const char* e = getenv("OMP_NUM_THREADS")) {
+        const int k = atoi(e);
+        if (k > 0) hw = std::min(hw, (unsigned)k);
+    }
+    return std::max(1u, hw);
+}
+
 void parallel_for(int64_t n, const std::function<void(int64_t, int64_t)>& fn) {
     if (n <= 0) return;
-    unsigned hw = std::thread::hardware_concurrency();
-    if (const char* e = getenv("AFAR_HOST_THREADS")) hw = (unsigned)atoi(e);
+    unsigned hw = host_threads();
     int nt = (int)std::max(1u, std::min(hw, 64u));
     const int64_t chunk = std::max<int64_t>(256, n / (int64_t)(nt * 16) + 1);
     if (nt == 1 || n <= chunk) {

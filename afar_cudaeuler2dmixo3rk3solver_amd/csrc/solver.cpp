@@ -68,7 +68,10 @@ class SolverImpl final : public DeviceSolver {
     using v2 = typename VT<R>::v2;
 
    public:
-    SolverImpl(const RefArrays& ra, const SolverConfig& cfg, const BatchIndex* bi) : ra_(ra) {
+    // ra and its device layout L (make_dev_layout(ra, cfg.tile)) are built by
+    // the caller, outside the library's lock, and moved in.
+    SolverImpl(RefArrays&& ra, DevLayout&& L, const SolverConfig& cfg, const BatchIndex* bi)
+        : ra_(std::move(ra)), L_(std::move(L)) {
         cfg_ = cfg;
         if (bi) {
             bi_ = *bi;
@@ -76,11 +79,10 @@ class SolverImpl final : public DeviceSolver {
                 if (bi_.face_off[k + 1] - bi_.face_off[k] < 256)
                     throw Error(1, "every mesh of a batch needs at least 256 internal faces");
         } else {
-            bi_.cell_off = {0, ra.nc};
-            bi_.face_off = {0, ra.nf};
-            bi_.bface_off = {0, ra.nb};
+            bi_.cell_off = {0, ra_.nc};
+            bi_.face_off = {0, ra_.nf};
+            bi_.bface_off = {0, ra_.nb};
         }
-        L_ = make_dev_layout(ra, cfg.tile);
         if (cfg.stream) {
             sh_.s = cfg.stream;  // the caller's stream: used, never destroyed
         } else {
@@ -565,6 +567,17 @@ class SolverImpl final : public DeviceSolver {
 
     void dalloc(DevMem& m, size_t bytes) { m.alloc(bytes, stream_); }
     void dzero(void* p, size_t bytes) { check_hip(hipMemsetAsync(p, 0, bytes, stream_), "hipMemsetAsync"); }
+    // n doubles uploaded in the kernels' precision: as they are in FP64,
+    // rounded to float through a host copy in FP32-mixed
+    R* put_as_R(DevMem& m, const double* src, size_t n) {
+        if constexpr (sizeof(R) == sizeof(double)) {
+            return reinterpret_cast<R*>(put(m, src, n));
+        } else {
+            std::vector<R> c(n);
+            for (size_t i = 0; i < n; ++i) c[i] = (R)src[i];
+            return put(m, c.data(), n);
+        }
+    }
     // allocation and upload in stream order; returns once src may be reused
     template <typename T> T* put(DevMem& m, const T* src, size_t n) {
         dalloc(m, sizeof(T) * n);
@@ -634,23 +647,15 @@ class SolverImpl final : public DeviceSolver {
         P_.ccnt = put(m_ccnt, L_.ccnt.data(), nc);
         P_.sten = put(m_sten, L_.sten.data(), L_.sten.size());
         P_.slot_face = put(m_slot_face, L_.slot_face.data(), nslot);
-        {
-            // face offsets in the kernels' precision (they use them in it only)
-            std::vector<R> d(2 * nslot);
-            for (size_t i = 0; i < d.size(); ++i) d[i] = (R)L_.slot_d[i];
-            P_.slot_d = reinterpret_cast<const v2*>(put(m_slot_d, d.data(), d.size()));
-        }
+        // face offsets in the kernels' precision (they use them in it only)
+        P_.slot_d = reinterpret_cast<const v2*>(put_as_R(m_slot_d, L_.slot_d.data(), 2 * nslot));
         P_.slot_e = reinterpret_cast<const double2*>(put(m_slot_e, L_.slot_e.data(), 2 * nslot));
         dalloc(m_slot_dr, sizeof(R) * nslot);
         P_.slot_dr = m_slot_dr.as<R>();
         dzero(P_.slot_dr, sizeof(R) * nslot);
         dalloc(m_lsq, sizeof(v2) * nslot);
         P_.lsq = m_lsq.as<v2>();
-        {
-            std::vector<R> rbf(L_.rbf.size());
-            for (size_t i = 0; i < rbf.size(); ++i) rbf[i] = (R)L_.rbf[i];
-            P_.rbf = put(m_rbf, rbf.data(), rbf.size());
-        }
+        P_.rbf = put_as_R(m_rbf, L_.rbf.data(), L_.rbf.size());
         P_.halo_ptr = put(m_halo_ptr, L_.halo_ptr.data(), L_.halo_ptr.size());
         {
             // one trailing id (cell 0) so that a tile without halo may read
@@ -688,12 +693,8 @@ class SolverImpl final : public DeviceSolver {
         if (L_.ngp > 4) throw Error(3, "more than four flux quadrature points");
         for (int g = 0; g < L_.ngp; ++g) P_.gpw[g] = L_.gpw[g];
         if (L_.ngp) {
-            std::vector<R> rg(L_.rbf_gp.size());
-            for (size_t i = 0; i < rg.size(); ++i) rg[i] = (R)L_.rbf_gp[i];
-            P_.rbf_gp = put(m_rbf_gp, rg.data(), rg.size());
-            std::vector<R> gd(L_.fgd.size());
-            for (size_t i = 0; i < gd.size(); ++i) gd[i] = (R)L_.fgd[i];
-            P_.fgd = reinterpret_cast<const v4*>(put(m_fgd, gd.data(), gd.size()));
+            P_.rbf_gp = put_as_R(m_rbf_gp, L_.rbf_gp.data(), L_.rbf_gp.size());
+            P_.fgd = reinterpret_cast<const v4*>(put_as_R(m_fgd, L_.fgd.data(), L_.fgd.size()));
             dalloc(m_fvg, sizeof(v4) * 2 * nf * L_.ngp);
             P_.fvg = m_fvg.as<v4>();
         }
@@ -916,9 +917,10 @@ class RelabeledSolver final : public DeviceSolver {
     int nc_, nf_, nb_;
 };
 
-std::unique_ptr<DeviceSolver> make_impl(const RefArrays& ra, const SolverConfig& cfg, const BatchIndex* bi) {
-    if (cfg.precision == 1) return std::unique_ptr<DeviceSolver>(new SolverImpl<float>(ra, cfg, bi));
-    return std::unique_ptr<DeviceSolver>(new SolverImpl<double>(ra, cfg, bi));
+std::unique_ptr<DeviceSolver> make_impl(RefArrays&& ra, DevLayout&& L, const SolverConfig& cfg, const BatchIndex* bi) {
+    if (cfg.precision == 1)
+        return std::unique_ptr<DeviceSolver>(new SolverImpl<float>(std::move(ra), std::move(L), cfg, bi));
+    return std::unique_ptr<DeviceSolver>(new SolverImpl<double>(std::move(ra), std::move(L), cfg, bi));
 }
 
 }  // namespace
@@ -926,9 +928,14 @@ std::unique_ptr<DeviceSolver> make_impl(const RefArrays& ra, const SolverConfig&
 std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const SolverConfig& cfg,
                                                  const BatchIndex* bi) {
     if (ra.nc <= 0) throw Error(1, "empty mesh");
+    // host-side work (copies, relabelling, the device layout) runs before the
+    // library's lock is taken, so that solvers prepared on several threads
+    // overlap in it; the lock covers the device allocation and upload
     if (cfg.cellOrder == 1) {
+        RefArrays copy = ra;
+        DevLayout L = make_dev_layout(copy, cfg.tile);
         std::lock_guard<std::mutex> lk(runtime_mutex());
-        return make_impl(ra, cfg, bi);
+        return make_impl(std::move(copy), std::move(L), cfg, bi);
     }
     BatchIndex one;
     if (!bi) {
@@ -942,9 +949,10 @@ std::unique_ptr<DeviceSolver> make_device_solver(const RefArrays& ra, const Solv
     Relabel r = compact_tile_order(ra, bi ? *bi : one, group);
     std::unique_ptr<DeviceSolver> in;
     {
-        const RefArrays rl = relabel(ra, r);
+        RefArrays rl = relabel(ra, r);
+        DevLayout L = make_dev_layout(rl, cfg.tile);
         std::lock_guard<std::mutex> lk(runtime_mutex());
-        in = make_impl(rl, cfg, bi);
+        in = make_impl(std::move(rl), std::move(L), cfg, bi);
     }
     return std::unique_ptr<DeviceSolver>(new RelabeledSolver(std::move(in), std::move(r), ra.nc, ra.nf, ra.nb));
 }

@@ -26,6 +26,7 @@ import argparse
 import faulthandler
 import json
 import math
+import os
 import queue
 import sys
 import threading
@@ -140,9 +141,11 @@ class Server:
         return self.sc if req.kind == "corner" else self.sc_riemann
 
     def _create(self, req, mesh, W0):
-        """The request's solver, state on the device (preparation threads)."""
+        """The request's solver, state on the device, step graph captured
+        (preparation threads: the scheduler then only launches)."""
         g = BasicCUDASolver(mesh, self.schemes_of(req), W0)
         g.mesh = None  # the solver holds its own copy: the host mesh can go
+        g.evolve_auto(0)  # captures the step graph; no step is taken
         req.cells = mesh.nc
         return g
 
@@ -180,12 +183,17 @@ class Server:
         self._enqueue(req, g)
         return False
 
-    def _finish(self, req, g, t0):
+    def _finish(self, req, g, t0, reap=None):
+        """Result to the host; the solver is destroyed by `reap` (a thread
+        that frees device memory off the scheduler) or here."""
         try:
             req.result = g.conserved()
         finally:
-            g.close()
             req.times["finish"] = time.perf_counter() - t0
+            if reap is None:
+                g.close()
+            else:
+                reap(g)
 
     @staticmethod
     def _fail(req, exc, t0):
@@ -245,11 +253,23 @@ class Server:
             finally:
                 ready_q.put(None)  # always posted, so the scheduler's producer count reaches zero
 
+        reap_q = queue.Queue()
+
+        def reaper():  # destroys finished solvers: a destroy waits for the solver's stream and takes the
+            while True:  # library's lock, which the scheduler thread should not
+                g = reap_q.get()
+                if g is None:
+                    return
+                g.close()
+
         feeder = threading.Thread(target=arrivals, daemon=True)
         preps = [threading.Thread(target=preparer, daemon=True) for _ in range(self.prep_workers)]
+        reap_t = threading.Thread(target=reaper, daemon=True)
         feeder.start()
+        reap_t.start()
         for p in preps:
             p.start()
+        busy = self.sched_s = dict(start=0.0, advance=0.0, finish=0.0, idle=0.0)
 
         waiting = []   # prepared, not admitted (arrival order)
         active = []    # (request, solver, footprint)
@@ -279,24 +299,31 @@ class Server:
                 r.times["admitted"] = time.perf_counter() - t0
                 progressed = True
                 slots.release()
+                c0 = time.perf_counter()
                 try:
                     self._start(r, g, t0)
                 except Exception as e:
-                    g.close()
+                    reap_q.put(g)
                     self._fail(r, e, t0)
                     done.append(r)
                     continue
+                finally:
+                    busy["start"] += time.perf_counter() - c0
                 active.append((r, g, need))
                 used += need
                 self.peak_active = max(self.peak_active, len(active))
             still = []
             for r, g, need in active:
+                c0 = time.perf_counter()
                 try:
                     finished = g.idle() and self._advance(r, g)
+                    c1 = time.perf_counter()
+                    busy["advance"] += c1 - c0
                     if finished:
-                        self._finish(r, g, t0)
+                        self._finish(r, g, t0, reap_q.put)
+                        busy["finish"] += time.perf_counter() - c1
                 except Exception as e:
-                    g.close()
+                    reap_q.put(g)
                     self._fail(r, e, t0)
                     finished = True
                 if finished:
@@ -318,10 +345,14 @@ class Server:
                     except queue.Empty:
                         pass
                 else:
+                    c0 = time.perf_counter()
                     time.sleep(self.poll_s)
+                    busy["idle"] += time.perf_counter() - c0
         feeder.join()
         for p in preps:
             p.join()
+        reap_q.put(None)
+        reap_t.join()
         while not failed_q.empty():
             done.append(failed_q.get_nowait())
         return sorted(done, key=lambda r: r.rid), time.perf_counter() - t0
@@ -373,7 +404,15 @@ def main(argv=None):
     ap.add_argument("--gpu-precompute", action="store_true",
                     help="RBF precompute on the GPU, from the preparation threads (default: on the host)")
     ap.add_argument("--prep-workers", type=int, default=2, help="host threads preparing requests")
+    ap.add_argument("--host-threads", type=int, default=0,
+                    help="threads of each request's parallel host loops (0: the CPU share / prep workers)")
     args = ap.parse_args(argv)
+    # the preparation threads already run requests side by side: each request's own parallel loops
+    # get a share of the CPUs, not all of them (AFAR_HOST_THREADS, read by the library per loop)
+    cpus = len(os.sched_getaffinity(0))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cpus = min(cpus, max(1, int(os.environ["OMP_NUM_THREADS"])))
+    os.environ["AFAR_HOST_THREADS"] = str(args.host_threads or max(1, cpus // max(1, args.prep_workers)))
     reqs = request_stream(args.rate, args.duration, args.seed, args.nc_min, args.nc_max, args.t_end, args.res_tol,
                           args.max_steps)
     srv = Server(max_active=args.max_active, chunk=args.chunk, log=lambda m: print(m, flush=True),
@@ -415,10 +454,12 @@ def main(argv=None):
         rate=args.rate, duration=args.duration, res_tol_rel=args.res_tol, corner_step_cap=args.max_steps,
         riemann_t_end=args.t_end,
         nc_range=[args.nc_min, args.nc_max], wall_s=round(wall, 2),
+        host_threads_per_request=int(os.environ["AFAR_HOST_THREADS"]),
         p50_tts_s=round(float(np.percentile(tts, 50)), 3) if len(tts) else None,
         p99_tts_s=round(float(np.percentile(tts, 99)), 3) if len(tts) else None,
         cell_steps_per_s=round(cell_steps / wall, 1),
         Mcell_steps_per_s=round(cell_steps / wall / 1e6, 2), peak_active=srv.peak_active,
+        scheduler_s={k: round(v, 2) for k, v in srv.sched_s.items()},
         fp64_sample=len(errs), rho_L1_rel_vs_fp64_max=max(errs) if errs else None,
         rho_L1_rel_vs_fp64_median=float(np.median(errs)) if errs else None)), flush=True)
 
