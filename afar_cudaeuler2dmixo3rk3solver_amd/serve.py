@@ -112,13 +112,14 @@ class Server:
 
     Host threads prepare requests: mesh generation, precompute (its RBF part
     on the GPU with gpu_precompute) and the solver with its state on the
-    device (the library's calls are safe from several threads).  One
-    scheduler thread steps the solvers: it admits
+    device and its step graph captured (the library's calls are safe from
+    several threads).  One scheduler thread steps the solvers: it admits
     prepared requests while their footprint fits in the HBM budget and a slot
     is free, enqueues chunks of graph-launched steps on each active solver's
     stream without waiting, and polls the streams; a request whose stream is
     idle either gets its next chunk, or is finished (result copied back,
-    solver destroyed, memory and slot released)."""
+    memory and slot released, the solver handed to a reaper thread that
+    destroys it, so that the scheduler never waits on the library's lock)."""
 
     def __init__(self, precision="mixed", hbm_budget=64 << 30, max_active=8, chunk=50, prep_workers=2,
                  log=None, poll_s=2e-4, gpu_precompute=False, max_prepared=None):
