@@ -143,3 +143,18 @@ def test_reconstruct_constant_field(tri_mesh):
     assert np.all(orc.smooth_flags == 0)
     assert np.array_equal(orc.final_types, (nb_faces >= 2).astype(np.int8))
     assert np.all(orc.a["limiter"] == 1.0)  # BJ(1) = 1
+
+
+def test_precompute_independent_of_host_threads(afar, monkeypatch):
+    """The host precompute's parallel loops give the same stencils and
+    coefficients bit for bit whatever the thread count (AFAR_HOST_THREADS)."""
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    out = []
+    for threads in ("1", "3", "8"):
+        monkeypatch.setenv("AFAR_HOST_THREADS", threads)
+        m, _ = cases.riemann_case(n=40, seed=4)
+        m.precompute(afar.schemes(cOrder=7, fOrder=2))
+        out.append((m.stencils(), m.rbf_basis()))
+    for (st, rb) in out[1:]:
+        for a, b in zip(st + rb, out[0][0] + out[0][1]):
+            assert np.array_equal(a, b)
