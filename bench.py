@@ -8,9 +8,23 @@ with KFVS/Roe; all-FP64; CFL 0.5.  The run is first advanced (untimed) to
 t = 0.15, the middle of the t = 0.3 run, where the waves cover much of the
 domain; then W warm-up steps and K timed steps.  A step = CFL time-step
 reduction + one SSP-RK3 step (3 stages), time step kept on the device,
-replayed as a hipGraph; inputs resident in HBM.  Prints one JSON line.
+replayed as a hipGraph; inputs resident in HBM.  Boundary cells take
+first-order face states (the boundaryFirstOrder extension; the unmodified
+algorithm loses positivity at the boundary of this mesh), stated in config.
+Prints one JSON line:
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--case riemann|corner] [--precision fp64|mixed]
+  value          cell-RK3-steps per second over the K timed steps (Mcell)
+  roofline       per RK3 step: the pipeline's compulsory bytes (B_alg per cell
+                 x cells) over the measured step time against 8 TB/s; traffic =
+                 HBM bytes per step from the committed rocprofv3 counter
+                 summaries (profiles/riemann1000_*_hbm_traffic.txt); the
+                 dominant kernel's own figures beside it
+  cpu_baseline   the FP64 oracle (OpenMP, the host's CPUs) on the same mesh
+                 from the same state for about 15 s
+  accuracy       a fresh GPU solve from the oracle's starting state against
+                 the oracle after the same number of steps (FP64: bitwise)
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--case riemann|corner|batch] [--precision fp64|mixed]
 """
 import argparse
 import json
