@@ -50,12 +50,15 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b / span) * span + (b & 7) * G + (b >> 3) % G;
 }
 
+// exp and erfc of KFVS: the library's own FP64 functions; the FP32 path
+// rounds their value at the float argument to float (a result that depends
+// on no vendor math library, so a host restatement reproduces it).
 template <typename R> __device__ __forceinline__ R kexp(R x);
 template <> __device__ __forceinline__ double kexp<double>(double x) { return afar_exp(x); }
-template <> __device__ __forceinline__ float kexp<float>(float x) { return expf(x); }
+template <> __device__ __forceinline__ float kexp<float>(float x) { return (float)afar_exp((double)x); }
 template <typename R> __device__ __forceinline__ R kerfc(R x);
 template <> __device__ __forceinline__ double kerfc<double>(double x) { return afar_erfc(x); }
-template <> __device__ __forceinline__ float kerfc<float>(float x) { return erfcf(x); }
+template <> __device__ __forceinline__ float kerfc<float>(float x) { return (float)afar_erfc((double)x); }
 
 // ---------------------------------------------------------------- K3 least squares
 

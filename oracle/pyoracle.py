@@ -1,6 +1,7 @@
 """ORACLE wrapper -- test infrastructure only (tests/, __graft_entry__.smoke(),
 bench.py's CPU baseline).  ctypes binding of liboracle.so, the FP64 host
-restatement of the reference kernels (afar_oracle.cpp).
+restatement of the reference kernels (afar_oracle.cpp) and the FP32-mixed
+restatement of the library's mixed path (afar_oracle_mixed.cpp).
 
 ``OracleState`` owns reference-layout arrays (as exported by
 Mesh.reference_arrays) plus the device-only work arrays of the reference
@@ -63,6 +64,17 @@ def load(path):
     L.oracle_exp.restype = d
     L.oracle_erfc.argtypes = [d]
     L.oracle_erfc.restype = d
+    if hasattr(L, "oracle_mixed_evolve"):  # absent from single-file builds (the mutation tests')
+        L.oracle_mixed_adjust_time_step.argtypes = [pf, d, d, d]
+        L.oracle_mixed_adjust_time_step.restype = d
+        L.oracle_mixed_reconstruct.argtypes = [pf]
+        L.oracle_mixed_bvd.argtypes = [pf]
+        L.oracle_mixed_evaluate_flux.argtypes = [pf, d, d]
+        L.oracle_mixed_evolve.argtypes = [pf, d, d, d, C.c_int]
+        f = C.c_float
+        L.oracle_mixed_central_flux.argtypes = [P(f)] + [f] * 10 + [d, f, f]
+        L.oracle_mixed_roe_flux.argtypes = [P(f)] + [f] * 12 + [d, d, f, f, f]
+        L.oracle_mixed_kfvs_flux.argtypes = [P(f)] + [f] * 10 + [d, d, f, f, f]
     return L
 
 
@@ -80,11 +92,16 @@ def _p(a, t=C.c_double):
 
 
 class OracleState:
-    """Reference-layout state advanced by the oracle kernels."""
+    """Reference-layout state advanced by the oracle kernels: the FP64
+    restatement, or with ``mixed`` the FP32-mixed one (face-centre fluxes
+    only)."""
 
     RK_LITERAL = ((1.0, 0.75, 0.333333), (0.0, 0.25, 0.666667), (1.0, 0.25, 0.666667))
 
-    def __init__(self, ref, R, Cv, boundary_first_order=False):
+    def __init__(self, ref, R, Cv, boundary_first_order=False, mixed=False):
+        if mixed and "RBFgp" in ref:
+            raise ValueError("the FP32-mixed oracle covers face-centre fluxes only")
+        self.mixed = bool(mixed)
         s = ref["sizes"]
         nc, nf, nb, maxS, maxC = s["nc"], s["nf"], s["nb"], s["maxS"], s["maxC"]
         self.R, self.Cv = R, Cv
@@ -127,26 +144,34 @@ class OracleState:
         self.f = f
         lib().oracle_least_square_matrix(C.byref(f))
 
+    def _fn(self, name):
+        return getattr(lib(), ("oracle_mixed_" if self.mixed else "oracle_") + name)
+
     def adjust_time_step(self, CFL):
-        return lib().oracle_adjust_time_step(C.byref(self.f), self.R, self.Cv, CFL)
+        return self._fn("adjust_time_step")(C.byref(self.f), self.R, self.Cv, CFL)
 
     def evolve(self, dt, exact_rk=False):
-        lib().oracle_evolve(C.byref(self.f), self.R, self.Cv, dt, int(exact_rk))
+        self._fn("evolve")(C.byref(self.f), self.R, self.Cv, dt, int(exact_rk))
 
     def run(self, nsteps, CFL, exact_rk=False):
         dts = np.zeros(nsteps)
-        lib().oracle_run(C.byref(self.f), self.R, self.Cv, CFL, nsteps, int(exact_rk), _p(dts))
+        if not self.mixed:
+            lib().oracle_run(C.byref(self.f), self.R, self.Cv, CFL, nsteps, int(exact_rk), _p(dts))
+            return dts
+        for i in range(nsteps):  # the same loop as oracle_run
+            dts[i] = self.adjust_time_step(CFL)
+            self.evolve(dts[i], exact_rk)
         return dts
 
     # single kernels, for kernel-level checks
     def reconstruct(self):
-        lib().oracle_reconstruct(C.byref(self.f))
+        self._fn("reconstruct")(C.byref(self.f))
 
     def bvd(self):
-        lib().oracle_bvd(C.byref(self.f))
+        self._fn("bvd")(C.byref(self.f))
 
     def flux(self):
-        lib().oracle_evaluate_flux(C.byref(self.f), self.R, self.Cv)
+        self._fn("evaluate_flux")(C.byref(self.f), self.R, self.Cv)
 
     def residual_update(self, dt, stage=0):
         lib().oracle_evaluate_residual(C.byref(self.f))
@@ -186,4 +211,26 @@ def kfvs_flux(rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, R, Cv, magSf, Sfx,
 def central_flux(rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Cv, Sfx, Sfy):
     out = np.zeros(4)
     lib().oracle_central_flux(_p(out), rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Cv, Sfx, Sfy)
+    return out
+
+
+# The same three in float (the FP32-mixed oracle); float arguments are rounded
+# to float by ctypes, R and Cv stay double.
+def mixed_central_flux(rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Cv, Sfx, Sfy):
+    out = np.zeros(4, np.float32)
+    lib().oracle_mixed_central_flux(_p(out, C.c_float), rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Cv, Sfx, Sfy)
+    return out
+
+
+def mixed_roe_flux(rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, dP, dRho, R, Cv, magSf, Sfx, Sfy, base):
+    out = np.array(base, dtype=np.float32)
+    lib().oracle_mixed_roe_flux(_p(out, C.c_float), rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, dP, dRho, R, Cv,
+                                magSf, Sfx, Sfy)
+    return out
+
+
+def mixed_kfvs_flux(rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, R, Cv, magSf, Sfx, Sfy, base):
+    out = np.array(base, dtype=np.float32)
+    lib().oracle_mixed_kfvs_flux(_p(out, C.c_float), rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, R, Cv, magSf, Sfx,
+                                 Sfy)
     return out

@@ -1,0 +1,277 @@
+"""Pins of the FP32-mixed oracle (oracle/afar_oracle_mixed.cpp) on the CPU.
+
+The mixed oracle is the checker of the device's mixed path (bitwise, in
+tests/test_gpu_mixed_parity.py), so it is pinned here on its own:
+
+- kernel by kernel it equals the FP64 oracle (itself pinned by the golden
+  fixture, the hand-computed switch pins and the mutation tests) up to float
+  round-off, when both read the same float-representable inputs: the point
+  fluxes (central, Roe and KFVS, on random faces, expansions and sonic
+  points), the reconstruction, the BVD switch and the face-flux pass;
+- a full RK3 step agrees with the FP64 step to float accuracy;
+- a uniform flow stays uniform to float round-off (free-stream preservation);
+- the hand-computed switch pins of tests/test_cpu_switch_pins.py (run there
+  with mixed=True) hold.
+
+Each pin_* function raises AssertionError on a disagreement; the mutation
+test below plants plausible slips in a copy of the mixed oracle and checks
+that some pin catches each one.
+"""
+import concurrent.futures
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def f32(a):
+    return a.astype(np.float32).astype(np.float64)
+
+
+def _states(n=96, steps=5):
+    """FP64 and mixed oracle states of the four-quadrant Riemann problem on a
+    jittered triangle/quad mesh after `steps` FP64 steps, both holding the
+    same float-representable primitive state (ghosts included)."""
+    import afar_cudaeuler2dmixo3rk3solver_amd as A
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    m, W0 = cases.riemann_case(n=n, seed=1)
+    sc = A.schemes(cOrder=7, fOrder=2, maxCo=0.5)
+    m.precompute(sc)
+    ref = m.reference_arrays(sc, W0)
+    d = O.OracleState(ref, sc.R, sc.Cv)
+    d.run(steps, sc.maxCo)
+    x = O.OracleState(ref, sc.R, sc.Cv, mixed=True)
+    d.a["FD"][:] = f32(d.a["FD"])
+    for k in ("FD", "cFD", "cFDold"):
+        x.a[k][:] = d.a[k]
+    return d, x, sc
+
+
+def _copy_stage_inputs(src, dsts):
+    """The reconstruction's outputs of src, rounded to float, into each of dsts."""
+    for k in ("gradFD", "limiter", "faceFD"):
+        v = f32(src.a[k])
+        for t in dsts:
+            t.a[k][:] = v
+    for t in dsts:
+        t.a["shockIndicator"][:] = src.a["shockIndicator"]
+
+
+def _close(a, b, tol, frac=1.0):
+    """|a - b| <= tol * (max |b| of the column) for at least frac of the rows."""
+    scale = np.abs(b).reshape(len(b), -1).max(axis=0)
+    err = np.abs(a - b).reshape(len(b), -1) / np.maximum(scale, 1e-300)
+    ok = (err <= tol).all(axis=1)
+    return ok.mean() >= frac, ok.mean(), err.max()
+
+
+def pin_reconstruct_matches_fp64():
+    d, x, _ = _states()
+    d.reconstruct()
+    x.reconstruct()
+    good, share, worst = _close(x.a["gradFD"], d.a["gradFD"], 1e-5)  # linear sums: no branch
+    assert good, ("gradients", share, worst)
+    good, share, worst = _close(x.a["faceFD"].reshape(-1, 8), d.a["faceFD"].reshape(-1, 8), 1e-5, 0.99)
+    assert good, ("face values", share, worst)
+    # the ratio 0.5 d1 / d2 of a face whose d2 is near SMALL carries the float
+    # cancellation error of d2 (1e-7 absolute), so a few limiters differ more
+    good, share, worst = _close(x.a["limiter"], d.a["limiter"], 1e-4, 0.97)
+    assert good, ("limiters", share, worst)
+    assert (x.smooth_flags == d.smooth_flags).mean() >= 0.995
+    assert d.smooth_flags.any()
+
+
+def pin_bvd_matches_fp64():
+    d, x, _ = _states()
+    d.reconstruct()
+    _copy_stage_inputs(d, [d, x])
+    d.bvd()
+    x.bvd()
+    same = x.final_types == d.final_types
+    assert same.mean() >= 0.995, same.mean()
+    assert (d.final_types < 0).any() and (d.final_types > 0).any() and (d.final_types == 0).any()
+    lx, ld = x.a["limiter"][same], d.a["limiter"][same]
+    assert np.abs(lx - ld).max() <= 1e-6, np.abs(lx - ld).max()
+
+
+def pin_flux_matches_fp64():
+    """Roe, KFVS, central part and boundary fluxes from identical inputs."""
+    d, x, _ = _states()
+    d.reconstruct()
+    d.bvd()
+    _copy_stage_inputs(d, [d, x])
+    d.flux()
+    x.flux()
+    nf = d.s["nf"]
+    Fd, Fx = d.a["Flux"], x.a["Flux"]
+    own, nei = d.a["neighbour"][:, 0], d.a["neighbour"][:, 1]
+    types = d.a["shockIndicator"][1::2]
+    shock = (types[own] < 0) | (types[nei] < 0)
+    assert shock.any() and (~shock).any()
+    scale = np.abs(Fd[:nf]).max(axis=0)
+    for sel, name in ((shock, "KFVS"), (~shock, "Roe")):
+        err = (np.abs(Fx[:nf][sel] - Fd[:nf][sel]) / scale).max()
+        assert err <= 2e-5, (name, err)
+    # boundary faces: double arithmetic on the same float ghost states
+    assert np.abs(Fx[nf:] - Fd[nf:]).max() <= 1e-14 * max(np.abs(Fd[nf:]).max(), 1.0)
+
+
+def pin_step_matches_fp64():
+    d, x, sc = _states(steps=3)
+    dt_d = d.adjust_time_step(sc.maxCo)
+    dt_x = x.adjust_time_step(sc.maxCo)
+    assert dt_d == dt_x  # the same float-representable states, the same double arithmetic
+    d.evolve(dt_d)
+    x.evolve(dt_d)
+    rel = np.abs(x.U - d.U).sum(axis=0) / np.abs(d.U).sum(axis=0)
+    assert rel.max() <= 1e-5, rel
+
+
+def pin_free_stream():
+    import afar_cudaeuler2dmixo3rk3solver_amd as A
+    m = A.Mesh.riemann(n=30, jitter=0.2, seed=4, split_p=0.5)
+    sc = A.schemes(cOrder=7, fOrder=2, maxCo=0.5)
+    m.precompute(sc)
+    W0 = np.tile([0.9, 0.7, -0.4, 0.9 / (1.2 * sc.R)], (m.nc, 1))
+    x = O.OracleState(m.reference_arrays(sc, W0), sc.R, sc.Cv, mixed=True)
+    U0 = x.U.copy()
+    x.run(10, sc.maxCo)
+    assert np.abs(x.U - U0).max() <= 1e-5 * np.abs(U0).max()
+
+
+def _face_samples(rng, n, R):
+    """Float-representable face states: random pairs, symmetric expansions
+    about rest (the contact wave's entropy fix active) and sonic expansions
+    (the acoustic waves' fix active), on random unit normals."""
+    out = []
+    for i in range(n):
+        pL, pR = rng.uniform(0.2, 3.0, 2)
+        TL, TR = rng.uniform(0.5, 2.0, 2) / R
+        th = rng.uniform(0, 2 * np.pi)
+        mag = rng.uniform(0.01, 0.1)
+        nx, ny = np.cos(th), np.sin(th)
+        c = np.sqrt(1.4 * R * TL)
+        if i % 3 == 0:
+            uL, uR = rng.uniform(-2, 2, 2) * c, rng.uniform(-2, 2, 2) * c
+        elif i % 3 == 1:
+            a, t = rng.uniform(0.2, 1.5) * c, rng.uniform(-0.5, 0.5) * c
+            uL = np.array([-a * nx - t * ny, -a * ny + t * nx])
+            uR = np.array([a * nx - t * ny, a * ny + t * nx])
+        else:
+            a = rng.uniform(0.1, 0.5) * c
+            uL, uR = np.array([(c - a) * nx, (c - a) * ny]), np.array([(c + a) * nx, (c + a) * ny])
+        v = np.float32([pL, pR, uL[0], uR[0], uL[1], uR[1], TL, TR, mag, mag * nx, mag * ny]).astype(np.float64)
+        out.append(v)
+    return out
+
+
+def pin_point_fluxes_match_fp64():
+    """The float central, Roe and KFVS face fluxes against the FP64 oracle's
+    on the same float inputs: within 1e-5 of the sample's flux scale
+    (measured worst 7e-7)."""
+    R, Cv = 287.0, 287.0 / 0.4
+    rng = np.random.default_rng(3)
+    for pL, pR, uxL, uxR, uyL, uyR, TL, TR, mag, Sfx, Sfy in _face_samples(rng, 600, R):
+        rhoL = float(np.float32(pL) / (np.float32(R) * np.float32(TL)))
+        rhoR = float(np.float32(pR) / (np.float32(R) * np.float32(TR)))
+        dP, dRho = float(np.float32(pR - pL)), float(np.float32(rhoR - rhoL))
+        args = (rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR)
+        cen = O.central_flux(*args, Cv, Sfx, Sfy)
+        cen32 = O.mixed_central_flux(*args, Cv, Sfx, Sfy)
+        scale = np.abs(cen).max()
+        assert np.abs(cen32 - cen).max() <= 1e-5 * scale
+        roe = O.roe_flux(*args, dP, dRho, R, Cv, mag, Sfx, Sfy, base=cen)
+        roe32 = O.mixed_roe_flux(*args, dP, dRho, R, Cv, mag, Sfx, Sfy, base=cen32)
+        assert np.abs(roe32 - roe).max() <= 1e-5 * (scale + np.abs(roe).max()), ("Roe", args, roe32, roe)
+        kf = O.kfvs_flux(*args, R, Cv, mag, Sfx, Sfy, base=cen)
+        kf32 = O.mixed_kfvs_flux(*args, R, Cv, mag, Sfx, Sfy, base=cen32)
+        assert np.abs(kf32 - kf).max() <= 1e-5 * (scale + np.abs(kf).max()), ("KFVS", args, kf32, kf)
+
+
+PINS = [pin_point_fluxes_match_fp64, pin_reconstruct_matches_fp64, pin_bvd_matches_fp64, pin_flux_matches_fp64,
+        pin_step_matches_fp64, pin_free_stream]
+
+
+@pytest.mark.parametrize("pin", PINS, ids=[p.__name__ for p in PINS])
+def test_mixed_oracle_pin(afar, pin):
+    pin()
+
+
+def test_mixed_rejects_flux_points(afar):
+    import afar_cudaeuler2dmixo3rk3solver_amd as A
+    m = A.Mesh.riemann(n=6, seed=1)
+    sc = A.schemes(cOrder=7, fOrder=2, faceQuadrature=1, boundaryFirstOrder=True)
+    m.precompute(sc)
+    ref = m.reference_arrays(sc, np.tile([1.0, 0.0, 0.0, 1.0 / sc.R], (m.nc, 1)))
+    with pytest.raises(ValueError, match="face-centre"):
+        O.OracleState(ref, sc.R, sc.Cv, mixed=True)
+
+
+# (name, text in the mixed oracle, replacement): slips a restatement could make
+MUTATIONS = [
+    ("bvd_factor_0.6", "BVD_FACTOR = (R)0.5", "BVD_FACTOR = (R)0.6"),
+    ("valid_lim_factor_0.5", "VALID_LIM_FACTOR = (R)0.9", "VALID_LIM_FACTOR = (R)0.5"),
+    ("limiter_cap_1", "R yy = fmin((R)2, ", "R yy = fmin((R)1, "),
+    ("smoothness_threshold_2", "f->shockIndicator[2 * (size_t)celli] = smoothnessValue > (R)1;",
+     "f->shockIndicator[2 * (size_t)celli] = smoothnessValue > (R)2;"),
+    ("transform_selection_inverted", "const int isShock = finalType < 0;", "const int isShock = finalType > 0;"),
+    ("d1_branches_swapped", "(R)(phif < phic) * (phiMin - phic) + (R)(phif >= phic) * (phiMax - phic)",
+     "(R)(phif < phic) * (phiMax - phic) + (R)(phif >= phic) * (phiMin - phic)"),
+    ("gradient_components_swapped", "g[GUXY] += M1 * dphi;\n            g[GUYY] += M2 * dphi;",
+     "g[GUXY] += M2 * dphi;\n            g[GUYY] += M1 * dphi;"),
+    ("rbf_reads_wrong_variable", "Uxf += coef * w_of(f, st[ci], 1);", "Uxf += coef * w_of(f, st[ci], 2);"),
+    ("tvd_state_index", "const R pLT = wL[0] + lL[0] * (gL[GPX] * dxL + gL[GPY] * dyL);",
+     "const R pLT = wL[0] + lL[0] * (gL[GPX] * dxL + gL[GPX] * dyL);"),
+    ("roe_r2_sign", "const R r2 = dRho - dP / (cT * cT);", "const R r2 = dRho + dP / (cT * cT);"),
+    ("roe_contact_entropy_fix_dropped",
+     "(R)(l2 >= eps) * l2 + (R)(l2 < eps) * ((l2 * l2 + eps * eps) / ((R)2 * eps + (R)1e-30))", "l2"),
+    ("roe_acoustic_entropy_fix_dropped",
+     "(R)(l1 >= eps) * l1 + (R)(l1 < eps) * ((l1 * l1 + eps * eps) / ((R)2 * eps + (R)1e-30))", "l1"),
+    ("kfvs_internal_energy_dof", "const R Kd = (R)2 * Cv / Rg - (R)2;", "const R Kd = (R)2 * Cv / Rg;"),
+    ("kfvs_exp_sign", "(R)0.5 * kexp(-lamL * uxLl * uxLl)", "(R)0.5 * kexp(lamL * uxLl * uxLl)"),
+    ("central_energy_pressure_dropped",
+     "out[3] = (R)0.5 * out[0] * (CvR * TL + (R)0.5 * (uxL * uxL + uyL * uyL) + pL / rhoL) +",
+     "out[3] = (R)0.5 * out[0] * (CvR * TL + (R)0.5 * (uxL * uxL + uyL * uyL)) +"),
+    ("boundary_flux_pressure_dropped", "out[1] = rhoFlux * bUx + bp * Sfx;", "out[1] = rhoFlux * bUx;"),
+    ("cfl_sound_speed_gamma", "sqrt((Rgas / Cv + 1.0) * Rgas *", "sqrt(Rgas *"),
+]
+
+
+def _build(tmp, name, old, new):
+    src = open(os.path.join(ROOT, "oracle", "afar_oracle_mixed.cpp")).read()
+    assert src.count(old) == 1, name
+    path = os.path.join(tmp, name + "_mixed.cpp")
+    with open(path, "w") as f:
+        f.write(src.replace(old, new))
+    so = os.path.join(tmp, "lib" + name + ".so")
+    # -iquote: both sources include the oracle header and the library's math header from oracle/
+    cmd = ["g++", "-O1", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-shared", "-iquote",
+           os.path.join(ROOT, "oracle"), "-o", so, os.path.join(ROOT, "oracle", "afar_oracle.cpp"), path]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return so
+
+
+@pytest.fixture(scope="module")
+def mutated_libs(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("mixed_oracle_mutations"))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        futs = {m[0]: ex.submit(_build, tmp, *m) for m in MUTATIONS}
+        return {k: f.result() for k, f in futs.items()}
+
+
+@pytest.mark.parametrize("name", [m[0] for m in MUTATIONS])
+def test_pins_catch_mixed_mutation(afar, mutated_libs, monkeypatch, name):
+    from tests import test_cpu_switch_pins as S
+    monkeypatch.setattr(O, "_lib", O.load(mutated_libs[name]))
+    failed = []
+    for pin in PINS + [lambda p=p: p(mixed=True) for p in S.PINS]:
+        try:
+            pin()
+        except AssertionError:
+            failed.append(pin)
+    assert failed, "no pin catches the mutation %s" % name

@@ -16,11 +16,14 @@ arithmetic and the reference's formulas, worked out in the comments:
 
 The pin_* functions raise AssertionError when the oracle disagrees;
 tests/test_cpu_oracle_mutations.py runs them against deliberately broken
-builds of the oracle to show that each catches its slip.
+builds of the oracle to show that each catches its slip.  Each pin also runs
+on the FP32-mixed oracle (mixed=True): the same expected values, with
+tolerances of float round-off where a value is not exact.
 """
 import ctypes as C
 
 import numpy as np
+import pytest
 
 from oracle import pyoracle as O
 
@@ -38,15 +41,16 @@ def _grid(labels=None, values=None, n=N, jitter=0.0, seed=1):
     return m, sc
 
 
-def _oracle(m, sc, field):
-    """Oracle state whose cell values are field(x, y) -> (nc, 4) at the cell
-    centres (mesh order); ghosts from the boundary table."""
+def _oracle(m, sc, field, mixed=False):
+    """Oracle state (FP64, or FP32-mixed with ``mixed``) whose cell values are
+    field(x, y) -> (nc, 4) at the cell centres (mesh order); ghosts from the
+    boundary table."""
     xy, _ = m.cells()
     nc = m.nc
     Wm = field(xy[:nc, 0], xy[:nc, 1])
     W0 = np.empty_like(Wm)
     W0[m.permutation()] = Wm
-    orc = O.OracleState(m.reference_arrays(sc, W0), sc.R, sc.Cv)
+    orc = O.OracleState(m.reference_arrays(sc, W0), sc.R, sc.Cv, mixed=mixed)
     return orc, xy
 
 
@@ -62,7 +66,7 @@ def _col(x, a, b):
     return np.where(x < a, 0, np.where(x < b, 1, 2))
 
 
-def pin_limiter_branches():
+def pin_limiter_branches(mixed=False):
     """Raw K5 limiters of the interior cell c at (0.45, 0.55).
 
     p: 0.9 left of c's column, 1.0 in it, 1.3 right of it.  gx = 0.4/(2h),
@@ -92,14 +96,15 @@ def pin_limiter_branches():
         uy = np.choose(col, [0.0, 0.0, 1e-6])
         return np.stack([p, ux, uy, 1.0 + 0.5 * y], axis=1)
 
-    orc, xy = _oracle(m, sc, field)
+    orc, xy = _oracle(m, sc, field, mixed)
     c = _cell_at(xy, nc, 0.45, 0.55)
     orc.reconstruct()
     lim = orc.a["limiter"][c]
-    assert abs(lim[0] - 0.5) < 1e-8, lim
-    assert abs(lim[1] - 0.5) < 1e-8, lim
+    tol = 1e-6 if mixed else 1e-8  # round-off of the float or double arithmetic
+    assert abs(lim[0] - 0.5) < tol, lim
+    assert abs(lim[1] - 0.5) < tol, lim
     assert lim[2] == 1.0, lim
-    assert abs(lim[3] - 1.0) < 1e-8, lim
+    assert abs(lim[3] - 1.0) < tol, lim
     # the SMALL range rule: Uy face values are the linear extrapolation
     fv = orc.a["faceFD"]
     cf = orc.a["cellFaces"][c][: orc.a["compactStencilSize"][c]]
@@ -107,10 +112,10 @@ def pin_limiter_branches():
         f, side = abs(sf) - 1, int(sf < 0)
         xf = orc.a["FACE"][f, 0]
         expect = 2.5e-7 * np.sign(xf - 0.45) if abs(xf - 0.45) > 1e-9 else 0.0
-        assert abs(fv[f, side, 2] - expect) < 1e-15, (sf, fv[f, side, 2], expect)
+        assert abs(fv[f, side, 2] - expect) < (1e-13 if mixed else 1e-15), (sf, fv[f, side, 2], expect)
 
 
-def pin_limiter_cap():
+def pin_limiter_cap(mixed=False):
     """The cap y <= 2 (cudaLimiter.h:47).  Corner cell c at (0.05, 0.05) with
     ghosts set by hand: pseudo-cells L at (0, 0.05) and B at (0.05, 0) (h/2
     away) carry the extremes, and boundary faces are not limited.  p: c = 20,
@@ -127,7 +132,7 @@ def pin_limiter_cap():
         p[(np.abs(x - 0.05) < 1e-9) & (np.abs(y - 0.15) < 1e-9)] = 14.6
         return np.stack([p, np.zeros_like(x), np.zeros_like(x), 1.0 + 0.5 * x + 0.25 * y], axis=1)
 
-    orc, xy = _oracle(m, sc, field)
+    orc, xy = _oracle(m, sc, field, mixed)
     c = _cell_at(xy, nc, 0.05, 0.05)
     FD = orc.a["FD"]
     for g in range(nc, len(FD)):
@@ -139,7 +144,7 @@ def pin_limiter_cap():
     assert orc.a["limiter"][c, 0] == 2.0, orc.a["limiter"][c]
 
 
-def pin_smoothness_kink():
+def pin_smoothness_kink(mixed=False):
     """Smoothness flag on p = 1 + s |x - 0.45| (kink at the centre of column
     4), T = p / R (rho = 1).  At the kink cell gx = 0, so phif = pc at every
     face; the east face gives |(1/2) pc + (1/2) pE - pc| / h = (s h/2)/h =
@@ -154,7 +159,7 @@ def pin_smoothness_kink():
             p = 1.0 + s * np.abs(x - 0.45)
             return np.stack([p, np.zeros_like(x), np.zeros_like(x), p / sc.R], axis=1)
 
-        orc, xy = _oracle(m, sc, field)
+        orc, xy = _oracle(m, sc, field, mixed)
         orc.reconstruct()
         flags = orc.smooth_flags
         x = xy[:nc, 0]
@@ -164,7 +169,7 @@ def pin_smoothness_kink():
         assert np.all(flags[inner & ~kink] == 0), s
 
 
-def _bvd_case(Tc, Tcur, Tnei, raw_T, flags_on=(), raw_other=0.3):
+def _bvd_case(Tc, Tcur, Tnei, raw_T, flags_on=(), raw_other=0.3, mixed=False):
     """BVD of the interior cell c at (0.45, 0.55) with hand-set inputs:
     zero gradients (so Tf_cur_TVD = Tc), the raw limiter (raw_other for p,
     Ux, Uy; raw_T for T), smoothness flags 1 on the cells at the positions
@@ -174,7 +179,7 @@ def _bvd_case(Tc, Tcur, Tnei, raw_T, flags_on=(), raw_other=0.3):
     m, sc = _grid()
     nc = m.nc
     orc, xy = _oracle(m, sc, lambda x, y: np.stack([np.ones_like(x), np.zeros_like(x), np.zeros_like(x),
-                                                    np.ones_like(x)], axis=1))
+                                                    np.ones_like(x)], axis=1), mixed)
     c = _cell_at(xy, nc, 0.45, 0.55)
     a = orc.a
     a["gradFD"][:] = 0.0
@@ -203,7 +208,7 @@ def _venkat(y):
     return min((y * y + y + 1e-10) / (y * y + 0.5 + 0.5 * y + 1e-10), 1.0)
 
 
-def pin_bvd_truth_table():
+def pin_bvd_truth_table(mixed=False):
     """Each factor of the BVD switch (cudaLimiter.h:431-451) on c's east face,
     with T_TVD = Tc (zero gradient), l = min(2 raw_T, 1):
       typeval   = |Tc - Tnei| < 0.5 |Tcur - Tnei|
@@ -227,22 +232,23 @@ def pin_bvd_truth_table():
         (1.45, 1.0, 2.0, 0.40, (E, D), -2),     # compact 1, extended 2: shock
     ]
     for Tc, Tcur, Tnei, raw_T, flags, expect in rows:
-        got, lim = _bvd_case(Tc, Tcur, Tnei, raw_T, flags)
+        got, lim = _bvd_case(Tc, Tcur, Tnei, raw_T, flags, mixed=mixed)
         assert got == expect, (Tc, Tcur, Tnei, raw_T, flags, got, expect)
         # the transform of cudaLimiter.h:453-461: Venkatakrishnan for shock types
         tr = _venkat if expect < 0 else _bj
-        assert abs(lim[0] - tr(0.3)) < 1e-15 and abs(lim[3] - tr(raw_T)) < 1e-15, (expect, lim)
+        tol = 1e-7 if mixed else 1e-15
+        assert abs(lim[0] - tr(0.3)) < tol and abs(lim[3] - tr(raw_T)) < tol, (expect, lim)
     # the two transforms at the raw value 0.3: BJ 0.6, Venkatakrishnan 0.39/0.74
     assert abs(_venkat(0.3) - 0.39 / 0.74) < 1e-9 and _bj(0.3) == 0.6
 
 
-def pin_bvd_corner_cells():
+def pin_bvd_corner_cells(mixed=False):
     """A cell with two boundary faces takes type 1 (cudaLimiter.h:419-429)
     whatever its faces hold; a cell with one does not."""
     m, sc = _grid()
     nc = m.nc
     orc, xy = _oracle(m, sc, lambda x, y: np.stack([np.ones_like(x), np.zeros_like(x), np.zeros_like(x),
-                                                    np.ones_like(x)], axis=1))
+                                                    np.ones_like(x)], axis=1), mixed)
     orc.reconstruct()
     orc.bvd()
     t = orc.final_types
@@ -252,7 +258,7 @@ def pin_bvd_corner_cells():
     assert t[_cell_at(xy, nc, 0.45, 0.45)] == 0
 
 
-def pin_cfl_brute_force():
+def pin_cfl_brute_force(mixed=False):
     """adjustTimeStep against a numpy evaluation of cudaFieldOperation.h:
     145-157 over every internal face of jittered 3 x 3 and 6 x 6 meshes with
     random states: dt = min(1e5, min_f CFL dr |S| / ((1 - a)|U_o.S| +
@@ -268,12 +274,12 @@ def pin_cfl_brute_force():
             return np.stack([rng.uniform(0.5, 2.0, k), rng.uniform(-3, 3, k), rng.uniform(-3, 3, k),
                              rng.uniform(0.5, 4.0, k)], axis=1)
 
-        orc, xy = _oracle(m, sc, field)
+        orc, xy = _oracle(m, sc, field, mixed)
         a = orc.a
         nf = orc.s["nf"]
         own, nei = a["neighbour"][:, 0], a["neighbour"][:, 1]
         F = a["FACE"][:nf]
-        W = a["FD"]
+        W = a["FD"].astype(np.float32).astype(np.float64) if mixed else a["FD"]  # the states the bound reads
         xo, xn = xy[own], xy[nei] + a["faceShift"]
         dr = np.sqrt(((xn - xo) ** 2).sum(axis=1))
         alpha = np.sqrt(((F[:, :2] - xo) ** 2).sum(axis=1)) / dr
@@ -292,7 +298,7 @@ def pin_cfl_brute_force():
         assert abs(swapped.min() - expect) > 1e-6 * expect
 
 
-def pin_boundary_ghosts():
+def pin_boundary_ghosts(mixed=False):
     """Ghost states of every boundary label from the adjacent cell's state
     w = (p, Ux, Uy, T) (src/CUDA/BCs/2D/*.h): 0 (p, 0, 0, 1); 1 (p, 0, 0, T);
     2 (p, U - (U.n) n, T), so U_g.n = 0 and U_g.t = U.t; 3 (1, 3, 0, 1);
@@ -308,7 +314,7 @@ def pin_boundary_ghosts():
             return np.stack([rng.uniform(0.5, 2.0, k), rng.uniform(-3, 3, k), rng.uniform(-3, 3, k),
                              rng.uniform(0.5, 4.0, k)], axis=1)
 
-        orc, xy = _oracle(m, sc, field)
+        orc, xy = _oracle(m, sc, field, mixed)
         a = orc.a
         nf = orc.s["nf"]
         a["FD"][nc:] = -99.0
@@ -338,7 +344,7 @@ def pin_boundary_ghosts():
         assert set(np.unique(a["boundaryFacesType"])) == set(labels)
 
 
-def pin_transformed_limiters_bounded():
+def pin_transformed_limiters_bounded(mixed=False):
     """After K6 every limiter lies in [0, 1] (both transforms are min(., 1) of
     a non-negative value), on a jittered triangle/quad Riemann run."""
     from afar_cudaeuler2dmixo3rk3solver_amd import cases
@@ -346,7 +352,7 @@ def pin_transformed_limiters_bounded():
     m, W0 = cases.riemann_case(n=24, seed=1)
     sc = A.schemes(cOrder=7, fOrder=2, maxCo=0.5)
     m.precompute(sc)
-    orc = O.OracleState(m.reference_arrays(sc, W0), sc.R, sc.Cv)
+    orc = O.OracleState(m.reference_arrays(sc, W0), sc.R, sc.Cv, mixed=mixed)
     orc.run(5, sc.maxCo)
     orc.reconstruct()
     raw = orc.a["limiter"].copy()
@@ -361,33 +367,38 @@ PINS = [pin_limiter_branches, pin_limiter_cap, pin_smoothness_kink, pin_bvd_trut
         pin_cfl_brute_force, pin_boundary_ghosts, pin_transformed_limiters_bounded]
 
 
-def test_limiter_branches(afar):
-    pin_limiter_branches()
+@pytest.fixture(params=[False, True], ids=["fp64", "mixed"])
+def mixed(request):
+    return request.param
 
 
-def test_limiter_cap(afar):
-    pin_limiter_cap()
+def test_limiter_branches(afar, mixed):
+    pin_limiter_branches(mixed)
 
 
-def test_smoothness_kink(afar):
-    pin_smoothness_kink()
+def test_limiter_cap(afar, mixed):
+    pin_limiter_cap(mixed)
 
 
-def test_bvd_truth_table(afar):
-    pin_bvd_truth_table()
+def test_smoothness_kink(afar, mixed):
+    pin_smoothness_kink(mixed)
 
 
-def test_bvd_corner_cells(afar):
-    pin_bvd_corner_cells()
+def test_bvd_truth_table(afar, mixed):
+    pin_bvd_truth_table(mixed)
 
 
-def test_cfl_brute_force(afar):
-    pin_cfl_brute_force()
+def test_bvd_corner_cells(afar, mixed):
+    pin_bvd_corner_cells(mixed)
 
 
-def test_boundary_ghosts(afar):
-    pin_boundary_ghosts()
+def test_cfl_brute_force(afar, mixed):
+    pin_cfl_brute_force(mixed)
 
 
-def test_transformed_limiters_bounded(afar):
-    pin_transformed_limiters_bounded()
+def test_boundary_ghosts(afar, mixed):
+    pin_boundary_ghosts(mixed)
+
+
+def test_transformed_limiters_bounded(afar, mixed):
+    pin_transformed_limiters_bounded(mixed)
