@@ -157,6 +157,8 @@ SIGNATURES = {
                                        _VP], C.c_int),
     "afar_selftest_math": ([_D, _D, C.c_int32], C.c_int),
     "afar_host_math": ([_D, _D, C.c_int32], C.c_int),
+    "afar_selftest_math_f32": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32], C.c_int),
+    "afar_host_math_f32": ([C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32], C.c_int),
     "afar_device_info": ([C.c_char_p, C.c_int32, _I32, _I64], C.c_int),
 }
 

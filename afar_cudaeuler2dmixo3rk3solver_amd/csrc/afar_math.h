@@ -16,6 +16,13 @@
 //   afar_hypot sqrt(x*x + y*y); the reference calls CUDA's hypot
 //              (src/CUDA/cudaEuler2DmixO3RK3/cudaFieldOperation.h:145-146,
 //              cudaLimiter.h:261) -- one definition is shared by every caller.
+//   afar_expf, afar_erfcf
+//              the same two methods in IEEE single precision for the FP32-mixed
+//              flux: the float split of ln 2 (ln2_hi with 15 significant bits,
+//              so n * ln2_hi is exact for |n| < 512), the degree-7 Taylor
+//              polynomial (truncation error < 6e-9 on |r| <= ln2/2), and
+//              Schonfelder's series to degree 13 in float (the terms dropped
+//              sum to < 1e-9); exp(-x^2) with a 12-bit high part of x.
 #pragma once
 #include <stdint.h>
 
@@ -96,4 +103,64 @@ AFAR_HD double afar_erfc(double x) {
         r = afar_exp_negsq(ax) * (g / (1.0 + 2.0 * ax));
     }
     return (x < 0.0) ? 2.0 - r : r;
+}
+
+// ---------------------------------------------------------------- single precision
+
+AFAR_HD uint32_t afar_bitsf(float x) { uint32_t u; __builtin_memcpy(&u, &x, 4); return u; }
+AFAR_HD float afar_from_bitsf(uint32_t u) { float x; __builtin_memcpy(&x, &u, 4); return x; }
+
+// 2^n for -126 <= n <= 127 (exact).
+AFAR_HD float afar_pow2if(int n) { return afar_from_bitsf((uint32_t)(n + 127) << 23); }
+
+AFAR_HD float afar_expf(float x) {
+    if (x != x) return x;                          // NaN
+    if (x > 89.0f) return afar_from_bitsf(0x7f800000u);
+    if (x < -104.0f) return 0.0f;                  // exp(-104) < half the smallest subnormal
+    const float log2e = 1.44269504f;
+    const float ln2_hi = 0.693145751953125f;       // 0x3f317200: 15 significant bits
+    const float ln2_lo = 1.42860677e-06f;          // ln 2 - ln2_hi
+    const float t = x * log2e;
+    const float n = (t >= 0.0f) ? (float)(int)(t + 0.5f) : -(float)(int)(0.5f - t);
+    const float r = (x - n * ln2_hi) - n * ln2_lo;
+    float p = 1.0f / 5040.0f;                      // 1/7!
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    int ni = (int)n;
+    if (ni > 127) {                                // close to overflow: split the scaling
+        p = p * 2.0f;
+        ni -= 1;
+    }
+    if (ni < -125) {                               // result may be subnormal: one rounding, at the end
+        p = p * afar_pow2if(-100);                 // exact (p >= 0.7 stays normal)
+        ni += 100;
+    }
+    return p * afar_pow2if(ni);
+}
+
+// exp(-x*x) for x >= 0 without the rounding error of x*x.
+AFAR_HD float afar_exp_negsqf(float x) {
+    const float xh = afar_from_bitsf(afar_bitsf(x) & 0xfffff000u);  // 12 significant bits: xh*xh exact
+    const float d = (x - xh) * (x + xh);                            // x^2 - xh^2
+    return afar_expf(-(xh * xh)) * afar_expf(-d);
+}
+
+AFAR_HD float afar_erfcf(float x) {
+    if (x != x) return x;
+    const float ax = (x < 0.0f) ? -x : x;
+    float r;
+    if (ax > 10.1f) {
+        r = 0.0f;                                  // erfc(10.1) < half the smallest subnormal
+    } else {
+        const float t = (ax - (float)AFAR_ERFCX_K) / (ax + (float)AFAR_ERFCX_K);
+        float g;
+        AFAR_ERFCX_CLENSHAW_F(t, g);
+        r = afar_exp_negsqf(ax) * (g / (1.0f + 2.0f * ax));
+    }
+    return (x < 0.0f) ? 2.0f - r : r;
 }

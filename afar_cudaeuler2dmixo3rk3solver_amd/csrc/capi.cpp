@@ -1101,6 +1101,43 @@ int afar_selftest_math(const double* x, double* out, int32_t n) {
     });
 }
 
+int afar_selftest_math_f32(const float* x, float* out, int32_t n) {
+    return guard([&] {
+        need(n >= 0 && x && out, "bad arguments");
+        std::lock_guard<std::mutex> lk(runtime_mutex());  // allocations, a device-wide wait, frees
+        float *dx = nullptr, *dout = nullptr;
+        check_hip(hipMalloc(&dx, sizeof(float) * (n + 1)), "hipMalloc");
+        hipError_t e = hipMalloc(&dout, sizeof(float) * 4 * (n + 1));
+        if (e != hipSuccess) {
+            (void)hipFree(dx);
+            check_hip(e, "hipMalloc");
+        }
+        e = hipMemcpy(dx, x, sizeof(float) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            launch_math_selftest_f32(dx, dout, n, nullptr);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(float) * 4 * n, hipMemcpyDeviceToHost);
+        (void)hipFree(dx);
+        (void)hipFree(dout);
+        check_hip(e, "selftest");
+    });
+}
+
+int afar_host_math_f32(const float* x, float* out, int32_t n) {
+    return guard([&] {
+        need(n >= 0 && x && out, "bad arguments");
+        for (int i = 0; i < n; ++i) {
+            const float v = x[i];
+            out[4 * i] = afar_expf(v);
+            out[4 * i + 1] = afar_erfcf(v);
+            out[4 * i + 2] = sqrtf(v < 0 ? -v : v);
+            out[4 * i + 3] = 1.0f / v;
+        }
+    });
+}
+
 int afar_host_math(const double* x, double* out, int32_t n) {
     return guard([&] {
         need(n >= 0 && x && out, "bad arguments");

@@ -119,6 +119,7 @@ void launch_update(const DevPtrs<R>& p, double Rg, double gamma, double dt, cons
 // Raises each reconstruction kernel's dynamic-LDS limit to lds minus its static LDS.
 template <typename R> hipError_t set_lds_limits(size_t lds);
 void launch_math_selftest(const double* x, double* out, int n, hipStream_t s);
+void launch_math_selftest_f32(const float* x, float* out, int n, hipStream_t s);
 // The CFL time step over the caller's device arrays in the reference's layout
 // (FD, CELL, FACE, neighbour, optional FSHIFT) into *deltaT (device memory),
 // on stream s; allocates nothing (its partial minima are a static device array).

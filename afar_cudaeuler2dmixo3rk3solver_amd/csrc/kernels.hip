@@ -50,15 +50,15 @@ __device__ __forceinline__ int xcd_tile(int b, int nblocks) {
     return (b / span) * span + (b & 7) * G + (b >> 3) % G;
 }
 
-// exp and erfc of KFVS: the library's own FP64 functions; the FP32 path
-// rounds their value at the float argument to float (a result that depends
-// on no vendor math library, so a host restatement reproduces it).
+// exp and erfc of KFVS: the library's own functions (afar_math.h) in the
+// working precision, which depend on no vendor math library, so the host
+// restatements reproduce them bit for bit.
 template <typename R> __device__ __forceinline__ R kexp(R x);
 template <> __device__ __forceinline__ double kexp<double>(double x) { return afar_exp(x); }
-template <> __device__ __forceinline__ float kexp<float>(float x) { return (float)afar_exp((double)x); }
+template <> __device__ __forceinline__ float kexp<float>(float x) { return afar_expf(x); }
 template <typename R> __device__ __forceinline__ R kerfc(R x);
 template <> __device__ __forceinline__ double kerfc<double>(double x) { return afar_erfc(x); }
-template <> __device__ __forceinline__ float kerfc<float>(float x) { return (float)afar_erfc((double)x); }
+template <> __device__ __forceinline__ float kerfc<float>(float x) { return afar_erfcf(x); }
 
 // ---------------------------------------------------------------- K3 least squares
 
@@ -1379,8 +1379,22 @@ __global__ void k_math_selftest(const double* x, double* out, int n) {
     out[4 * i + 3] = 1.0 / v;
 }
 
+__global__ void k_math_selftest_f32(const float* x, float* out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = x[i];
+    out[4 * i] = afar_expf(v);
+    out[4 * i + 1] = afar_erfcf(v);
+    out[4 * i + 2] = sqrtf(v < 0 ? -v : v);
+    out[4 * i + 3] = 1.0f / v;
+}
+
 void launch_math_selftest(const double* x, double* out, int n, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_math_selftest, dim3((n + 255) / 256), dim3(256), 0, s, x, out, n);
+}
+
+void launch_math_selftest_f32(const float* x, float* out, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_math_selftest_f32, dim3((n + 255) / 256), dim3(256), 0, s, x, out, n);
 }
 
 }  // namespace afar

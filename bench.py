@@ -22,7 +22,9 @@ Prints one JSON line:
   cpu_baseline   the FP64 oracle (OpenMP, the host's CPUs) on the same mesh
                  from the same state for about 15 s
   accuracy       a fresh GPU solve from the oracle's starting state against
-                 the oracle after the same number of steps (FP64: bitwise)
+                 the FP64 oracle after the same number of steps (FP64:
+                 bitwise); FP32-mixed also against the FP32-mixed oracle
+                 (bitwise)
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--case riemann|corner|batch] [--precision fp64|mixed]
 """
@@ -256,11 +258,23 @@ def cpu_baseline(mesh, sc, W0, budget_s=15.0, check_steps=100):
     return entry, check
 
 
-def accuracy_vs_oracle(mesh, sc, W0, check):
+def mixed_oracle_state(mesh, sc, W0, steps):
+    """The FP32-mixed oracle's conserved state after `steps` steps (CFL time
+    step of each) from W0: the checker of the mixed path."""
+    from oracle import pyoracle as O
+    ref = mesh.reference_arrays(sc, W0)
+    orc = O.OracleState(ref, sc.R, sc.Cv, boundary_first_order=bool(sc.boundaryFirstOrder), mixed=True)
+    del ref
+    orc.run(steps, sc.maxCo)
+    return orc.U.copy()
+
+
+def accuracy_vs_oracle(mesh, sc, W0, check, U_mixed=None):
     """A fresh GPU solver from the oracle's starting state W0 (both built by
     the same export), as many device-time-step steps as the oracle ran:
-    relative L1 of each conserved variable (density first) against the
-    oracle's state, and whether they are equal bit for bit."""
+    relative L1 of each conserved variable (density first) against the FP64
+    oracle's state, and whether they are equal bit for bit; with U_mixed (the
+    FP32-mixed oracle's state after as many steps), the same against it."""
     import afar_cudaeuler2dmixo3rk3solver_amd as A
     g = A.BasicCUDASolver(mesh, sc, W0)
     g.evolve_auto(check["steps"])
@@ -269,8 +283,13 @@ def accuracy_vs_oracle(mesh, sc, W0, check):
     g.close()
     Uo = check["U"]
     rel = np.abs(U - Uo).sum(axis=0) / np.abs(Uo).sum(axis=0)
-    return dict(reference="FP64 oracle (CPU), same mesh and starting state", steps=check["steps"],
-                rho_L1_rel=float(rel[0]), L1_rel=[float(v) for v in rel], bitwise_equal=bool(np.array_equal(U, Uo)))
+    out = dict(reference="FP64 oracle (CPU), same mesh and starting state", steps=check["steps"],
+               rho_L1_rel=float(rel[0]), L1_rel=[float(v) for v in rel], bitwise_equal=bool(np.array_equal(U, Uo)))
+    if U_mixed is not None:
+        relm = np.abs(U - U_mixed).sum(axis=0) / np.abs(U_mixed).sum(axis=0)
+        out["mixed_oracle"] = dict(reference="FP32-mixed oracle (CPU), same mesh, starting state and steps",
+                                   rho_L1_rel=float(relm[0]), bitwise_equal=bool(np.array_equal(U, U_mixed)))
+    return out
 
 
 def main():
@@ -469,8 +488,9 @@ def main():
     cpu, accuracy = None, None
     if not args.no_cpu:
         cpu, check = cpu_baseline(mesh, sc, W_start)
+        U_mixed = mixed_oracle_state(mesh, sc, W_start, check["steps"]) if args.precision == "mixed" else None
         gpu.close()  # the accuracy check's solver takes its place in memory
-        accuracy = accuracy_vs_oracle(mesh, sc, W_start, check)
+        accuracy = accuracy_vs_oracle(mesh, sc, W_start, check, U_mixed)
     out = {
         "metric": "Mcell-RK3-steps/s", "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_events / args.steps, 4),

@@ -390,6 +390,10 @@ int afar_adjust_time_step_noalloc(const double* FD, const double* CELL, const do
 int afar_selftest_math(const double* x, double* out, int32_t n);
 /* Same functions on the host. */
 int afar_host_math(const double* x, double* out, int32_t n);
+/* The same four in single precision, as the FP32-mixed flux path evaluates
+ * them (exp and erfc: afar_expf, afar_erfcf), on the device and on the host. */
+int afar_selftest_math_f32(const float* x, float* out, int32_t n);
+int afar_host_math_f32(const float* x, float* out, int32_t n);
 /* Name and compute units of the current device. */
 int afar_device_info(char* name, int32_t cap, int32_t* cus, int64_t* hbm_bytes);
 

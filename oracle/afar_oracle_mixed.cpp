@@ -12,8 +12,9 @@
 // floats they are computed as.  The conserved state, the residual gather, the
 // RK update, the boundary ghosts (formed from the double primitive state) and
 // the CFL time step (over float states) stay in double; those parts are the
-// FP64 oracle's own functions.  KFVS uses exp and erfc in double, rounded to
-// float, as the device's mixed path does.  Built with -ffp-contract=off.
+// FP64 oracle's own functions.  KFVS uses the library's float exp and erfc
+// (afar_expf, afar_erfcf), as the device's mixed path does.  Built with
+// -ffp-contract=off.
 #include <cmath>
 
 #include "afar_oracle.h"
@@ -37,8 +38,8 @@ const R SMALL = (R)1e-5, VSMALL = (R)1e-10, BVD_FACTOR = (R)0.5, VALID_LIM_FACTO
 
 inline R w_of(const oracle_fields* f, int c, int k) { return (R)f->FD[4 * (size_t)c + k]; }  // float state
 
-R kexp(R x) { return (R)afar_exp((double)x); }
-R kerfc(R x) { return (R)afar_erfc((double)x); }
+R kexp(R x) { return afar_expf(x); }
+R kerfc(R x) { return afar_erfcf(x); }
 
 void limiter_process(R& limiter, R& phif, R phic, R phiMax, R phiMin, R gx, R gy, R dx, R dy) {
     phif = phic + (gx * dx + gy * dy);
@@ -402,6 +403,9 @@ void oracle_mixed_evaluate_flux(oracle_fields* f, double Rgas, double Cv) {
         out[3] = rhoFlux * (Cv * bT + 0.5 * (bUx * bUx + bUy * bUy) + bp / rho);
     }
 }
+
+float oracle_expf(float x) { return afar_expf(x); }
+float oracle_erfcf(float x) { return afar_erfcf(x); }
 
 void oracle_mixed_evolve(oracle_fields* f, double Rgas, double Cv, double deltaT, int exactRK) {
     const double gamma = Rgas / Cv + 1.0;

@@ -75,6 +75,9 @@ def load(path):
         L.oracle_mixed_central_flux.argtypes = [P(f)] + [f] * 10 + [d, f, f]
         L.oracle_mixed_roe_flux.argtypes = [P(f)] + [f] * 12 + [d, d, f, f, f]
         L.oracle_mixed_kfvs_flux.argtypes = [P(f)] + [f] * 10 + [d, d, f, f, f]
+        for name in ("oracle_expf", "oracle_erfcf"):
+            getattr(L, name).argtypes = [f]
+            getattr(L, name).restype = f
     return L
 
 

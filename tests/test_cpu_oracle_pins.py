@@ -1,6 +1,7 @@
 """Pins of the oracle's point functions against what mathematics fixes.
 
-- exp / erfc against mpmath (40 digits): <= 1 ulp and <= 8 ulp.
+- exp / erfc against mpmath (40 digits): <= 1 ulp and <= 8 ulp; their float
+  versions (the FP32-mixed flux) <= 1.5 ulp and <= 5 ulp of float.
 - Roe and KFVS flux consistency: F(U, U) equals the physical Euler flux.
 - Roe upwinding: for a supersonic face with all wave speeds positive and no
   entropy fix active, Roe(UL, UR) = F(UL) when the true jumps are used.
@@ -52,6 +53,61 @@ def test_erfc_against_mpmath():
     assert ulp_err(O.lib().oracle_erfc(0.0), 1.0) <= 1.0
     assert O.lib().oracle_erfc(30.0) == 0.0
     assert O.lib().oracle_erfc(-30.0) == 2.0
+
+
+def ulp_err_f32(a, ref):
+    """|a - ref| in units of the float spacing at ref (ref exact, in double)."""
+    if float(a) == ref:
+        return 0.0
+    return abs(float(a) - ref) / float(np.spacing(np.abs(np.float32(ref))))
+
+
+def test_expf_against_mpmath():
+    """The float exp of the FP32-mixed flux: <= 1.5 ulp of float (measured
+    1.06), overflow, underflow and the exact point."""
+    mp.mp.dps = 40
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-87, 88.7, 4000), rng.uniform(-1, 1, 2000)]).astype(np.float32)
+    worst = max(ulp_err_f32(O.lib().oracle_expf(float(x)), float(mp.exp(mp.mpf(float(x))))) for x in xs)
+    assert worst <= 1.5, worst
+    assert O.lib().oracle_expf(0.0) == 1.0
+    assert O.lib().oracle_expf(90.0) == float("inf")
+    assert O.lib().oracle_expf(-110.0) == 0.0
+    # subnormal results: within one subnormal spacing
+    for x in (-90.0, -95.5, -100.0, -103.0):
+        ref = float(mp.exp(mp.mpf(x)))
+        assert abs(O.lib().oracle_expf(x) - ref) <= 2.0 ** -149, x
+
+
+def test_erfcf_against_mpmath():
+    """The float erfc of the FP32-mixed flux: <= 5 ulp of float where the
+    result is a normal float (measured 3.9), the limits and the symmetry."""
+    mp.mp.dps = 40
+    rng = np.random.default_rng(4)
+    xs = np.concatenate([rng.uniform(-10, 9.2, 4000), rng.uniform(-1, 1, 1000),
+                         rng.normal(0, 3, 1000)]).astype(np.float32)
+    worst = 0.0
+    for x in xs:
+        ref = float(mp.erfc(mp.mpf(float(x))))
+        if ref >= 1.2e-38:
+            worst = max(worst, ulp_err_f32(O.lib().oracle_erfcf(float(x)), ref))
+    assert worst <= 5.0, worst
+    assert O.lib().oracle_erfcf(0.0) == 1.0
+    assert O.lib().oracle_erfcf(11.0) == 0.0 and O.lib().oracle_erfcf(-11.0) == 2.0
+    assert O.lib().oracle_erfcf(1.5) + O.lib().oracle_erfcf(-1.5) == pytest.approx(2.0, abs=2.5e-7)
+
+
+def test_library_float_math_equals_oracle(afar):
+    """The library's host evaluation of the float functions (the code the
+    device compiles) equals the oracle's, bit for bit."""
+    from afar_cudaeuler2dmixo3rk3solver_amd._lib import check, lib, ptr
+    rng = np.random.default_rng(5)
+    x = np.concatenate([rng.uniform(-100, 90, 3000), rng.uniform(-12, 12, 3000)]).astype(np.float32)
+    out = np.zeros((len(x), 4), np.float32)
+    check(lib().afar_host_math_f32(ptr(x, C.c_float), ptr(out, C.c_float), len(x)))
+    for i in range(0, len(x), 7):
+        assert out[i, 0] == np.float32(O.lib().oracle_expf(float(x[i])))
+        assert out[i, 1] == np.float32(O.lib().oracle_erfcf(float(x[i])))
 
 
 def physical_flux(rho, p, ux, uy, T, Sfx, Sfy):

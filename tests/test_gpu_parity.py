@@ -37,7 +37,7 @@ def gpu_state_mesh_order(gpu, nc):
 
 
 def test_math_bit_exact(afar):
-    say("case: device exp/erfc/sqrt/div vs host, 40000 points")
+    say("case: device exp/erfc/sqrt/div vs host, 40000 points (FP64) and 50000 (float)")
     import ctypes as C
     from afar_cudaeuler2dmixo3rk3solver_amd._lib import check, dptr, lib
     rng = np.random.default_rng(0)
@@ -51,6 +51,14 @@ def test_math_bit_exact(afar):
     for i, v in enumerate(x[:50]):
         assert host[i, 0] == O.lib().oracle_exp(v)
         assert host[i, 1] == O.lib().oracle_erfc(v)
+    # the float functions of the FP32-mixed flux
+    from afar_cudaeuler2dmixo3rk3solver_amd._lib import ptr
+    xf = np.concatenate([x, rng.uniform(-104, 90, 10000)]).astype(np.float32)
+    devf = np.zeros((len(xf), 4), np.float32)
+    hostf = np.zeros((len(xf), 4), np.float32)
+    check(lib().afar_selftest_math_f32(ptr(xf, C.c_float), ptr(devf, C.c_float), len(xf)))
+    check(lib().afar_host_math_f32(ptr(xf, C.c_float), ptr(hostf, C.c_float), len(xf)))
+    assert np.array_equal(devf.view(np.uint32), hostf.view(np.uint32))
 
 
 def test_stage_kernels_corner(afar):
