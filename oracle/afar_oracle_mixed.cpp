@@ -14,7 +14,9 @@
 // the CFL time step (over float states) stay in double; those parts are the
 // FP64 oracle's own functions.  KFVS uses the library's float exp and erfc
 // (afar_expf, afar_erfcf), as the device's mixed path does.  Built with
-// -ffp-contract=off.
+// -ffp-contract=off.  With flux points (ngp > 0, the faceQuadrature
+// extension) the point states and fluxes are float as well and their weighted
+// sum is formed in double, as on the device.
 #include <cmath>
 
 #include "afar_oracle.h"
@@ -232,6 +234,15 @@ void oracle_mixed_reconstruct(oracle_fields* f) {
                 o[1] = wc[1] + g[GUXX] * dx + g[GUYX] * dy;
                 o[2] = wc[2] + g[GUXY] * dx + g[GUYY] * dy;
                 o[3] = wc[3] + g[GTX] * dx + g[GTY] * dy;
+                for (int q = 0; q < f->ngp; ++q) {  // the same linear state at the flux points
+                    const double* d = &f->gpOffsets[4 * ((size_t)q * f->nf + s.face)];
+                    const R gx = (R)(s.isNei ? d[2] : d[0]), gy = (R)(s.isNei ? d[3] : d[1]);
+                    double* og = &f->faceGP[4 * (((size_t)q * f->nf + s.face) * 2 + s.isNei)];
+                    og[0] = wc[0] + g[GPX] * gx + g[GPY] * gy;
+                    og[1] = wc[1] + g[GUXX] * gx + g[GUYX] * gy;
+                    og[2] = wc[2] + g[GUXY] * gx + g[GUYY] * gy;
+                    og[3] = wc[3] + g[GTX] * gx + g[GTY] * gy;
+                }
             }
             f->shockIndicator[2 * (size_t)celli] = 0;
             continue;
@@ -269,6 +280,27 @@ void oracle_mixed_reconstruct(oracle_fields* f) {
             limiter_process(tmpT, phif, wc[3], TMax, TMin, g[GTX], g[GTY], dx, dy);
             o[3] = (R)(rT >= SMALL) * Tf + (R)(rT < SMALL) * phif;
             isNotTrouble = phif > (R)0;
+            // flux points: RBF sums with each point's coefficients, or the
+            // unlimited linear state where the compact range is flat
+            for (int q = 0; q < f->ngp; ++q) {
+                const double* ag = &f->RBFgp[(((size_t)q * f->nf + s.face) * 2 + s.isNei) * (size_t)(maxS + 1)];
+                R cg = (R)ag[0];
+                R pg = cg * wc[0], Uxg = cg * wc[1], Uyg = cg * wc[2], Tg = cg * wc[3];
+                for (int ci = 0; ci < S; ++ci) {
+                    cg = (R)ag[ci + 1];
+                    pg += cg * w_of(f, st[ci], 0);
+                    Uxg += cg * w_of(f, st[ci], 1);
+                    Uyg += cg * w_of(f, st[ci], 2);
+                    Tg += cg * w_of(f, st[ci], 3);
+                }
+                const double* d = &f->gpOffsets[4 * ((size_t)q * f->nf + s.face)];
+                const R gx = (R)(s.isNei ? d[2] : d[0]), gy = (R)(s.isNei ? d[3] : d[1]);
+                double* og = &f->faceGP[4 * (((size_t)q * f->nf + s.face) * 2 + s.isNei)];
+                og[0] = (R)(rP >= SMALL) * pg + (R)(rP < SMALL) * (wc[0] + (g[GPX] * gx + g[GPY] * gy));
+                og[1] = (R)(rUx >= SMALL) * Uxg + (R)(rUx < SMALL) * (wc[1] + (g[GUXX] * gx + g[GUYX] * gy));
+                og[2] = (R)(rUy >= SMALL) * Uyg + (R)(rUy < SMALL) * (wc[2] + (g[GUXY] * gx + g[GUYY] * gy));
+                og[3] = (R)(rT >= SMALL) * Tg + (R)(rT < SMALL) * (wc[3] + (g[GTX] * gx + g[GTY] * gy));
+            }
         }
         const R nt = (R)isNotTrouble;
         lim[0] = fmax(tmpP, (R)0) * nt;
@@ -331,63 +363,92 @@ void oracle_mixed_bvd(oracle_fields* f) {
     }
 }
 
-// Face-centre fluxes in float (stored exactly in Flux), boundary faces in
-// double from the float ghost states.
+// The float flux of face faceI from the states reconstructed at a point with
+// offsets (dxL, dyL) from the owner and (dxR, dyR) from the neighbour; fL, fR
+// are the high-order states there (cudaRiemannSolver.h:155-251).
+static void point_flux_f(const oracle_fields* f, int faceI, R Rg, R CvR, R gamma, R dxL, R dyL, R dxR, R dyR,
+                         const R* fL, const R* fR, R* out) {
+    const int own = f->neighbour[2 * faceI];
+    const int nei = f->neighbour[2 * faceI + 1];
+    const double* F = &f->FACE[5 * (size_t)faceI];
+    const int8_t to = f->shockIndicator[2 * (size_t)own + 1], tn = f->shockIndicator[2 * (size_t)nei + 1];
+    const int isTVD_L = to != 0, isTVD_R = tn != 0, isShock = to < 0 || tn < 0;
+    const R magSf = (R)F[2], Sfx = (R)F[3], Sfy = (R)F[4];
+    R wL[4], wR[4], lL[4], lR[4], gL[8], gR[8];
+    for (int k = 0; k < 4; ++k) {
+        wL[k] = w_of(f, own, k);
+        wR[k] = w_of(f, nei, k);
+        lL[k] = (R)f->limiter[4 * (size_t)own + k];
+        lR[k] = (R)f->limiter[4 * (size_t)nei + k];
+    }
+    for (int k = 0; k < 8; ++k) {
+        gL[k] = (R)f->gradFD[8 * (size_t)own + k];
+        gR[k] = (R)f->gradFD[8 * (size_t)nei + k];
+    }
+    const R pLT = wL[0] + lL[0] * (gL[GPX] * dxL + gL[GPY] * dyL);
+    const R uxLT = wL[1] + lL[1] * (gL[GUXX] * dxL + gL[GUYX] * dyL);
+    const R uyLT = wL[2] + lL[2] * (gL[GUXY] * dxL + gL[GUYY] * dyL);
+    const R TLT = wL[3] + lL[3] * (gL[GTX] * dxL + gL[GTY] * dyL);
+    const R rhoLT = pLT / (Rg * TLT);
+    const R pRT = wR[0] + lR[0] * (gR[GPX] * dxR + gR[GPY] * dyR);
+    const R uxRT = wR[1] + lR[1] * (gR[GUXX] * dxR + gR[GUYX] * dyR);
+    const R uyRT = wR[2] + lR[2] * (gR[GUXY] * dxR + gR[GUYY] * dyR);
+    const R TRT = wR[3] + lR[3] * (gR[GTX] * dxR + gR[GTY] * dyR);
+    const R rhoRT = pRT / (Rg * TRT);
+    const R pL = (R)(1 - isTVD_L) * fL[0] + (R)isTVD_L * pLT;
+    const R uxL = (R)(1 - isTVD_L) * fL[1] + (R)isTVD_L * uxLT;
+    const R uyL = (R)(1 - isTVD_L) * fL[2] + (R)isTVD_L * uyLT;
+    const R TL = (R)(1 - isTVD_L) * fL[3] + (R)isTVD_L * TLT;
+    const R rhoL = pL / (Rg * TL);
+    const R pR = (R)(1 - isTVD_R) * fR[0] + (R)isTVD_R * pRT;
+    const R uxR = (R)(1 - isTVD_R) * fR[1] + (R)isTVD_R * uxRT;
+    const R uyR = (R)(1 - isTVD_R) * fR[2] + (R)isTVD_R * uyRT;
+    const R TR = (R)(1 - isTVD_R) * fR[3] + (R)isTVD_R * TRT;
+    const R rhoR = pR / (Rg * TR);
+    const R dP = minimize_delta(pL, pR, pLT, pRT);
+    const R dRho = minimize_delta(rhoL, rhoR, rhoLT, rhoRT);
+    central_flux(out, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, CvR, Sfx, Sfy);
+    if (isShock)
+        kfvs_flux(out, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Rg, CvR, magSf, Sfx, Sfy);
+    else
+        roe_flux(out, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, dP, dRho, gamma, CvR, magSf, Sfx, Sfy);
+}
+
+// Face fluxes in float (face centre: stored exactly in Flux; flux points:
+// their weighted sum in double, in point order), boundary faces in double
+// from the float ghost states.
 void oracle_mixed_evaluate_flux(oracle_fields* f, double Rgas, double Cv) {
     const R Rg = (R)Rgas, CvR = (R)Cv, gamma = (R)((Cv + Rgas) / Cv);
 #pragma omp parallel for schedule(static)
     for (int faceI = 0; faceI < f->nf; ++faceI) {
-        const int own = f->neighbour[2 * faceI];
-        const int nei = f->neighbour[2 * faceI + 1];
-        const double* F = &f->FACE[5 * (size_t)faceI];
-        const double xn = f->CELL[2 * nei] + shx(f, faceI), yn = f->CELL[2 * nei + 1] + shy(f, faceI);
-        const R dxL = (R)(F[0] - f->CELL[2 * own]), dyL = (R)(F[1] - f->CELL[2 * own + 1]);
-        const R dxR = (R)(F[0] - xn), dyR = (R)(F[1] - yn);
-        const int8_t to = f->shockIndicator[2 * (size_t)own + 1], tn = f->shockIndicator[2 * (size_t)nei + 1];
-        const int isTVD_L = to != 0, isTVD_R = tn != 0, isShock = to < 0 || tn < 0;
-        const R magSf = (R)F[2], Sfx = (R)F[3], Sfy = (R)F[4];
-        R wL[4], wR[4], lL[4], lR[4], gL[8], gR[8], fL[4], fR[4];
-        for (int k = 0; k < 4; ++k) {
-            wL[k] = w_of(f, own, k);
-            wR[k] = w_of(f, nei, k);
-            lL[k] = (R)f->limiter[4 * (size_t)own + k];
-            lR[k] = (R)f->limiter[4 * (size_t)nei + k];
-            fL[k] = (R)f->faceFD[4 * (2 * (size_t)faceI) + k];
-            fR[k] = (R)f->faceFD[4 * (2 * (size_t)faceI + 1) + k];
+        double* out = &f->Flux[4 * (size_t)faceI];
+        if (f->ngp == 0) {
+            const int own = f->neighbour[2 * faceI];
+            const int nei = f->neighbour[2 * faceI + 1];
+            const double* F = &f->FACE[5 * (size_t)faceI];
+            const double xn = f->CELL[2 * nei] + shx(f, faceI), yn = f->CELL[2 * nei + 1] + shy(f, faceI);
+            const R dxL = (R)(F[0] - f->CELL[2 * own]), dyL = (R)(F[1] - f->CELL[2 * own + 1]);
+            const R dxR = (R)(F[0] - xn), dyR = (R)(F[1] - yn);
+            R fL[4], fR[4], fq[4];
+            for (int k = 0; k < 4; ++k) {
+                fL[k] = (R)f->faceFD[4 * (2 * (size_t)faceI) + k];
+                fR[k] = (R)f->faceFD[4 * (2 * (size_t)faceI + 1) + k];
+            }
+            point_flux_f(f, faceI, Rg, CvR, gamma, dxL, dyL, dxR, dyR, fL, fR, fq);
+            for (int k = 0; k < 4; ++k) out[k] = fq[k];
+            continue;
         }
-        for (int k = 0; k < 8; ++k) {
-            gL[k] = (R)f->gradFD[8 * (size_t)own + k];
-            gR[k] = (R)f->gradFD[8 * (size_t)nei + k];
+        for (int q = 0; q < f->ngp; ++q) {  // extension: weighted sum over the face's quadrature points
+            const double* d = &f->gpOffsets[4 * ((size_t)q * f->nf + faceI)];
+            R fL[4], fR[4], fq[4];
+            for (int k = 0; k < 4; ++k) {
+                fL[k] = (R)f->faceGP[4 * (((size_t)q * f->nf + faceI) * 2) + k];
+                fR[k] = (R)f->faceGP[4 * (((size_t)q * f->nf + faceI) * 2 + 1) + k];
+            }
+            point_flux_f(f, faceI, Rg, CvR, gamma, (R)d[0], (R)d[1], (R)d[2], (R)d[3], fL, fR, fq);
+            const double w = f->gpWeights[q];
+            for (int k = 0; k < 4; ++k) out[k] = q == 0 ? w * (double)fq[k] : out[k] + w * (double)fq[k];
         }
-        const R pLT = wL[0] + lL[0] * (gL[GPX] * dxL + gL[GPY] * dyL);
-        const R uxLT = wL[1] + lL[1] * (gL[GUXX] * dxL + gL[GUYX] * dyL);
-        const R uyLT = wL[2] + lL[2] * (gL[GUXY] * dxL + gL[GUYY] * dyL);
-        const R TLT = wL[3] + lL[3] * (gL[GTX] * dxL + gL[GTY] * dyL);
-        const R rhoLT = pLT / (Rg * TLT);
-        const R pRT = wR[0] + lR[0] * (gR[GPX] * dxR + gR[GPY] * dyR);
-        const R uxRT = wR[1] + lR[1] * (gR[GUXX] * dxR + gR[GUYX] * dyR);
-        const R uyRT = wR[2] + lR[2] * (gR[GUXY] * dxR + gR[GUYY] * dyR);
-        const R TRT = wR[3] + lR[3] * (gR[GTX] * dxR + gR[GTY] * dyR);
-        const R rhoRT = pRT / (Rg * TRT);
-        const R pL = (R)(1 - isTVD_L) * fL[0] + (R)isTVD_L * pLT;
-        const R uxL = (R)(1 - isTVD_L) * fL[1] + (R)isTVD_L * uxLT;
-        const R uyL = (R)(1 - isTVD_L) * fL[2] + (R)isTVD_L * uyLT;
-        const R TL = (R)(1 - isTVD_L) * fL[3] + (R)isTVD_L * TLT;
-        const R rhoL = pL / (Rg * TL);
-        const R pR = (R)(1 - isTVD_R) * fR[0] + (R)isTVD_R * pRT;
-        const R uxR = (R)(1 - isTVD_R) * fR[1] + (R)isTVD_R * uxRT;
-        const R uyR = (R)(1 - isTVD_R) * fR[2] + (R)isTVD_R * uyRT;
-        const R TR = (R)(1 - isTVD_R) * fR[3] + (R)isTVD_R * TRT;
-        const R rhoR = pR / (Rg * TR);
-        const R dP = minimize_delta(pL, pR, pLT, pRT);
-        const R dRho = minimize_delta(rhoL, rhoR, rhoLT, rhoRT);
-        R out[4];
-        central_flux(out, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, CvR, Sfx, Sfy);
-        if (isShock)
-            kfvs_flux(out, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, Rg, CvR, magSf, Sfx, Sfy);
-        else
-            roe_flux(out, rhoL, rhoR, pL, pR, uxL, uxR, uyL, uyR, TL, TR, dP, dRho, gamma, CvR, magSf, Sfx, Sfy);
-        for (int k = 0; k < 4; ++k) f->Flux[4 * (size_t)faceI + k] = out[k];
     }
     for (int b = 0; b < f->nb; ++b) {
         const double* F = &f->FACE[5 * ((size_t)f->nf + b)];

@@ -96,14 +96,11 @@ def _p(a, t=C.c_double):
 
 class OracleState:
     """Reference-layout state advanced by the oracle kernels: the FP64
-    restatement, or with ``mixed`` the FP32-mixed one (face-centre fluxes
-    only)."""
+    restatement, or with ``mixed`` the FP32-mixed one."""
 
     RK_LITERAL = ((1.0, 0.75, 0.333333), (0.0, 0.25, 0.666667), (1.0, 0.25, 0.666667))
 
     def __init__(self, ref, R, Cv, boundary_first_order=False, mixed=False):
-        if mixed and "RBFgp" in ref:
-            raise ValueError("the FP32-mixed oracle covers face-centre fluxes only")
         self.mixed = bool(mixed)
         s = ref["sizes"]
         nc, nf, nb, maxS, maxC = s["nc"], s["nf"], s["nb"], s["maxS"], s["maxC"]
