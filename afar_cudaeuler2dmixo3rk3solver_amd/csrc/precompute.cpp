@@ -275,69 +275,86 @@ void bar_rule(int fOrder, std::vector<double>& p, std::vector<double>& w) {
 
 }  // namespace
 
+// Gauss points of a triangle (vertices from cell_xy[b..b+2]) from the unit
+// rule (sp, sw), weights divided by the cell's 2D area.
+static void tri_points(const Mesh& m, int cell, const Vec2* sp, const double* sw, int n, Vec2* pts, double* w) {
+    const int b = m.cell_ptr[cell];
+    const double faceArea = m.V[cell] / m.depth;
+    const Vec2 p1 = m.cell_xy[b], p2 = m.cell_xy[b + 1], p3 = m.cell_xy[b + 2];
+    const double jacobi = fabs((p2.x - p1.x) * (p3.y - p1.y) - (p3.x - p1.x) * (p2.y - p1.y));
+    for (int k = 0; k < n; ++k) {
+        const double xi = sp[k].x, eta = sp[k].y;
+        pts[k] = Vec2{p1.x + (p2.x - p1.x) * xi + (p3.x - p1.x) * eta, p1.y + (p2.y - p1.y) * xi + (p3.y - p1.y) * eta};
+        w[k] = sw[k] * jacobi / faceArea;
+    }
+}
+
+// The same for a quadrilateral, bilinear map.
+static void quad_points(const Mesh& m, int cell, const Vec2* sp, const double* sw, int n, Vec2* pts, double* w) {
+    const int b = m.cell_ptr[cell];
+    const double faceArea = m.V[cell] / m.depth;
+    // sortRecPoints (d2GaussQuadrature.C:283-331): polar angle about the
+    // vertex mean, largest first, bubble sort
+    Vec2 pt[4];
+    double ref_x = 0.0, ref_y = 0.0;
+    for (int i = 0; i < 4; ++i) {
+        pt[i] = m.cell_xy[b + i];
+        ref_x += pt[i].x;
+        ref_y += pt[i].y;
+    }
+    ref_x /= 4.0;
+    ref_y /= 4.0;
+    double polar[4];
+    int idx[4] = {0, 1, 2, 3};
+    for (int i = 0; i < 4; ++i) {
+        const double x = pt[i].x - ref_x, y = pt[i].y - ref_y;
+        const double r = sqrt(x * x + y * y);
+        polar[i] = y >= 0 ? acos(x / r) : -acos(x / r);
+    }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3 - i; ++j)
+            if (polar[j] < polar[j + 1]) {
+                std::swap(polar[j], polar[j + 1]);
+                std::swap(idx[j], idx[j + 1]);
+            }
+    const double x1 = pt[idx[0]].x, x2 = pt[idx[1]].x, x3 = pt[idx[2]].x, x4 = pt[idx[3]].x;
+    const double y1 = pt[idx[0]].y, y2 = pt[idx[1]].y, y3 = pt[idx[2]].y, y4 = pt[idx[3]].y;
+    for (int k = 0; k < n; ++k) {
+        const double xi = sp[k].x, eta = sp[k].y;
+        const double b1 = 0.25 * (1.0 - xi) * (1.0 - eta), b2 = 0.25 * (1.0 - xi) * (1.0 + eta);
+        const double b3 = 0.25 * (1.0 + xi) * (1.0 + eta), b4 = 0.25 * (1.0 + xi) * (1.0 - eta);
+        pts[k] = Vec2{x1 * b1 + x2 * b2 + x3 * b3 + x4 * b4, y1 * b1 + y2 * b2 + y3 * b3 + y4 * b4};
+        const double dxdxi = -0.25 * x1 * (1.0 - eta) - 0.25 * x2 * (1.0 + eta) + 0.25 * x3 * (1.0 + eta) + 0.25 * x4 * (1.0 - eta);
+        const double dxdeta = -0.25 * x1 * (1.0 - xi) + 0.25 * x2 * (1.0 - xi) + 0.25 * x3 * (1.0 + xi) - 0.25 * x4 * (1.0 + xi);
+        const double dydxi = -0.25 * y1 * (1.0 - eta) - 0.25 * y2 * (1.0 + eta) + 0.25 * y3 * (1.0 + eta) + 0.25 * y4 * (1.0 - eta);
+        const double dydeta = -0.25 * y1 * (1.0 - xi) + 0.25 * y2 * (1.0 - xi) + 0.25 * y3 * (1.0 + xi) - 0.25 * y4 * (1.0 + xi);
+        const double jacobi = fabs(dxdxi * dydeta - dxdeta * dydxi);
+        w[k] = sw[k] * jacobi / faceArea;
+    }
+}
+
+static int cell_vertices(const Mesh& m, int cell) {
+    const int nv = m.cell_ptr[cell + 1] - m.cell_ptr[cell];
+    if (nv != 3 && nv != 4) throw Error(1, "problem cell appear with 2D face points number = " + std::to_string(nv));
+    return nv;
+}
+
 // Cell Gauss points in the cell's own frame, weights divided by the cell's 2D
 // area so that sum(w f) is the cell average (d2GaussQuadrature.C:23-150).
 void gauss_cell(const Mesh& m, int cell, int cOrder, std::vector<Vec2>& pts, std::vector<double>& w) {
-    const int b = m.cell_ptr[cell], nv = m.cell_ptr[cell + 1] - b;
-    const double faceArea = m.V[cell] / m.depth;
     std::vector<Vec2> sp;
     std::vector<double> sw;
-    if (nv == 3) {
+    const int nv = cell_vertices(m, cell);
+    if (nv == 3)
         tri_rule(cOrder, sp, sw);
-        const Vec2 p1 = m.cell_xy[b], p2 = m.cell_xy[b + 1], p3 = m.cell_xy[b + 2];
-        pts.resize(sp.size());
-        w.resize(sp.size());
-        const double jacobi = fabs((p2.x - p1.x) * (p3.y - p1.y) - (p3.x - p1.x) * (p2.y - p1.y));
-        for (size_t k = 0; k < sp.size(); ++k) {
-            const double xi = sp[k].x, eta = sp[k].y;
-            pts[k] = Vec2{p1.x + (p2.x - p1.x) * xi + (p3.x - p1.x) * eta, p1.y + (p2.y - p1.y) * xi + (p3.y - p1.y) * eta};
-            w[k] = sw[k] * jacobi / faceArea;
-        }
-    } else if (nv == 4) {
+    else
         rec_rule(cOrder, sp, sw);
-        // sortRecPoints (d2GaussQuadrature.C:283-331): polar angle about the
-        // vertex mean, largest first, bubble sort
-        Vec2 pt[4];
-        double ref_x = 0.0, ref_y = 0.0;
-        for (int i = 0; i < 4; ++i) {
-            pt[i] = m.cell_xy[b + i];
-            ref_x += pt[i].x;
-            ref_y += pt[i].y;
-        }
-        ref_x /= 4.0;
-        ref_y /= 4.0;
-        double polar[4];
-        int idx[4] = {0, 1, 2, 3};
-        for (int i = 0; i < 4; ++i) {
-            const double x = pt[i].x - ref_x, y = pt[i].y - ref_y;
-            const double r = sqrt(x * x + y * y);
-            polar[i] = y >= 0 ? acos(x / r) : -acos(x / r);
-        }
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3 - i; ++j)
-                if (polar[j] < polar[j + 1]) {
-                    std::swap(polar[j], polar[j + 1]);
-                    std::swap(idx[j], idx[j + 1]);
-                }
-        const double x1 = pt[idx[0]].x, x2 = pt[idx[1]].x, x3 = pt[idx[2]].x, x4 = pt[idx[3]].x;
-        const double y1 = pt[idx[0]].y, y2 = pt[idx[1]].y, y3 = pt[idx[2]].y, y4 = pt[idx[3]].y;
-        pts.resize(sp.size());
-        w.resize(sp.size());
-        for (size_t k = 0; k < sp.size(); ++k) {
-            const double xi = sp[k].x, eta = sp[k].y;
-            const double b1 = 0.25 * (1.0 - xi) * (1.0 - eta), b2 = 0.25 * (1.0 - xi) * (1.0 + eta);
-            const double b3 = 0.25 * (1.0 + xi) * (1.0 + eta), b4 = 0.25 * (1.0 + xi) * (1.0 - eta);
-            pts[k] = Vec2{x1 * b1 + x2 * b2 + x3 * b3 + x4 * b4, y1 * b1 + y2 * b2 + y3 * b3 + y4 * b4};
-            const double dxdxi = -0.25 * x1 * (1.0 - eta) - 0.25 * x2 * (1.0 + eta) + 0.25 * x3 * (1.0 + eta) + 0.25 * x4 * (1.0 - eta);
-            const double dxdeta = -0.25 * x1 * (1.0 - xi) + 0.25 * x2 * (1.0 - xi) + 0.25 * x3 * (1.0 + xi) - 0.25 * x4 * (1.0 + xi);
-            const double dydxi = -0.25 * y1 * (1.0 - eta) - 0.25 * y2 * (1.0 + eta) + 0.25 * y3 * (1.0 + eta) + 0.25 * y4 * (1.0 - eta);
-            const double dydeta = -0.25 * y1 * (1.0 - xi) + 0.25 * y2 * (1.0 - xi) + 0.25 * y3 * (1.0 + xi) - 0.25 * y4 * (1.0 + xi);
-            const double jacobi = fabs(dxdxi * dydeta - dxdeta * dydxi);
-            w[k] = sw[k] * jacobi / faceArea;
-        }
-    } else {
-        throw Error(1, "problem cell appear with 2D face points number = " + std::to_string(nv));
-    }
+    pts.resize(sp.size());
+    w.resize(sp.size());
+    if (nv == 3)
+        tri_points(m, cell, sp.data(), sw.data(), (int)sp.size(), pts.data(), w.data());
+    else
+        quad_points(m, cell, sp.data(), sw.data(), (int)sp.size(), pts.data(), w.data());
 }
 
 void face_gauss_rule(int fOrder, std::vector<double>& xi, std::vector<double>& w) { bar_rule(fOrder, xi, w); }
@@ -350,19 +367,23 @@ Vec2 face_gauss_point(const Mesh& m, int f, double xi) {
 // Boundary-face Gauss points; weights include the face height and are then
 // divided by |S_f| so that they average over the face
 // (d2GaussQuadrature.C:217-279 with basicCUDAsolver.cpp:446-453).
-void gauss_boundary_face(const Mesh& m, int b, int fOrder, std::vector<Vec2>& pts, std::vector<double>& w) {
-    std::vector<double> sp, sw;
-    bar_rule(fOrder, sp, sw);
+static void bface_points(const Mesh& m, int b, const double* sp, const double* sw, int n, Vec2* pts, double* w) {
     const int f = m.nf + b;
     const Vec2 p0 = m.fp0[f], p1 = m.fp1[f];
     const double jacobi = sqrt((0.5 * (p0.x - p1.x)) * (0.5 * (p0.x - p1.x)) + (0.5 * (p0.y - p1.y)) * (0.5 * (p0.y - p1.y)));
-    pts.resize(sp.size());
-    w.resize(sp.size());
-    for (size_t k = 0; k < sp.size(); ++k) {
+    for (int k = 0; k < n; ++k) {
         const double xi = sp[k];
         pts[k] = Vec2{0.5 * p0.x * (1 - xi) + 0.5 * p1.x * (1 + xi), 0.5 * p0.y * (1 - xi) + 0.5 * p1.y * (1 + xi)};
         w[k] = (sw[k] * jacobi * m.depth) / m.magSf[f];
     }
+}
+
+void gauss_boundary_face(const Mesh& m, int b, int fOrder, std::vector<Vec2>& pts, std::vector<double>& w) {
+    std::vector<double> sp, sw;
+    bar_rule(fOrder, sp, sw);
+    pts.resize(sp.size());
+    w.resize(sp.size());
+    bface_points(m, b, sp.data(), sw.data(), (int)sp.size(), pts.data(), w.data());
 }
 
 // ------------------------------------------------------------------ pseudo-inverse
@@ -381,26 +402,39 @@ void pinv_jacobi(int n, const double* A, double* G) {
 void gauss_tables(const Mesh& m, const Schemes& sc, std::vector<double>& pts, std::vector<double>& w,
                   std::vector<int64_t>& ptr) {
     const int nc = m.nc, nb = m.nb;
-    std::vector<std::vector<Vec2>> P(nc + nb);
-    std::vector<std::vector<double>> Wt(nc + nb);
+    // the unit rules once; a quadrilateral rule is only asked for when the mesh has one
+    std::vector<Vec2> tp, qp;
+    std::vector<double> tw, qw, bp, bw;
+    bool tri = false, quad = false;
+    ptr.assign(nc + nb + 1, 0);
+    for (int c = 0; c < nc; ++c) (cell_vertices(m, c) == 3 ? tri : quad) = true;
+    if (tri) tri_rule(sc.cOrder, tp, tw);
+    if (quad) rec_rule(sc.cOrder, qp, qw);
+    if (nb) bar_rule(sc.fOrder, bp, bw);
+    for (int i = 0; i < nc + nb; ++i) {
+        const int64_t n = i < nc ? (int64_t)(m.cell_ptr[i + 1] - m.cell_ptr[i] == 3 ? tp.size() : qp.size())
+                                 : (int64_t)bp.size();
+        ptr[i + 1] = ptr[i] + n;
+    }
+    std::vector<Vec2> P((size_t)ptr[nc + nb]);
+    w.resize((size_t)ptr[nc + nb]);
     parallel_for(nc + nb, [&](int64_t i0, int64_t i1) {
         for (int64_t i = i0; i < i1; ++i) {
-            if (i < nc)
-                gauss_cell(m, (int)i, sc.cOrder, P[i], Wt[i]);
+            Vec2* p = P.data() + ptr[i];
+            double* wi = w.data() + ptr[i];
+            if (i >= nc)
+                bface_points(m, (int)(i - nc), bp.data(), bw.data(), (int)bp.size(), p, wi);
+            else if (m.cell_ptr[i + 1] - m.cell_ptr[i] == 3)
+                tri_points(m, (int)i, tp.data(), tw.data(), (int)tp.size(), p, wi);
             else
-                gauss_boundary_face(m, (int)(i - nc), sc.fOrder, P[i], Wt[i]);
+                quad_points(m, (int)i, qp.data(), qw.data(), (int)qp.size(), p, wi);
         }
     });
-    ptr.assign(nc + nb + 1, 0);
-    for (int i = 0; i < nc + nb; ++i) ptr[i + 1] = ptr[i] + (int64_t)P[i].size();
-    pts.resize(2 * (size_t)ptr[nc + nb]);
-    w.resize((size_t)ptr[nc + nb]);
-    for (int i = 0; i < nc + nb; ++i)
-        for (size_t k = 0; k < P[i].size(); ++k) {
-            pts[2 * (ptr[i] + k)] = P[i][k].x;
-            pts[2 * (ptr[i] + k) + 1] = P[i][k].y;
-            w[ptr[i] + k] = Wt[i][k];
-        }
+    pts.resize(2 * P.size());
+    for (size_t k = 0; k < P.size(); ++k) {
+        pts[2 * k] = P[k].x;
+        pts[2 * k + 1] = P[k].y;
+    }
 }
 
 Precomputed precompute(const Mesh& m, const Schemes& sc) {
