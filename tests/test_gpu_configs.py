@@ -1,16 +1,19 @@
 """GPU: the batched, served and precision-study workloads against the oracle.
 
 - Batch: the three smallest of the 256 corner-flow instances (seed 2: Mach
-  and corner angle drawn per instance) advanced together as one FP64 batch
-  with boundaryFirstOrder, as the batched benchmark runs them; after 40
-  device-time-step RK3 steps each mesh's state and last time step equal the
-  oracle's (run on that mesh alone) bit for bit.
-- Serving: an FP64 corner request (step cap) and an FP64 Riemann request
-  (device clock to t_end, the last step clipped) served together; each
-  result equals the oracle's bit for bit.
+  and corner angle drawn per instance) advanced together as one batch with
+  boundaryFirstOrder, as the batched benchmark runs them, in FP64 and in
+  FP32-mixed (the benchmark's precision); after 40 device-time-step RK3
+  steps each mesh's state and last time step equal the oracle's of that
+  precision (run on that mesh alone) bit for bit.
+- Serving: a corner request (step cap) and a Riemann request (device clock
+  to t_end, the last step clipped) served together, in FP64 and in
+  FP32-mixed (the served precision); each result equals the oracle's of that
+  precision bit for bit.
 - Precision: the FP32-mixed path on the isentropic vortex (periodic jittered
   quads, cell-average data) against the all-FP64 result: 64^2 quads to
-  t = 10 against the FP64 oracle, and 512^2 quads to t = 10 against the FP64
+  t = 10 against the FP64 oracle (and bit for bit against the FP32-mixed
+  oracle on the same time steps), and 512^2 quads to t = 10 against the FP64
   GPU path (itself bitwise the oracle), where the density L1 difference is
   below 1e-6 relative.
 """
@@ -28,28 +31,31 @@ def say(msg):
     print(msg, flush=True)
 
 
-def test_batch_instances_equal_oracle(afar):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_instances_equal_oracle(afar, precision):
     from afar_cudaeuler2dmixo3rk3solver_amd.batch import BatchSolver, build_corner_instances, corner_instances
     params = sorted(corner_instances(n=256, seed=2), key=lambda p: p["nx"] * p["ny"])[:3]
-    say("case: batch of the 3 smallest corner instances (seed 2), FP64, 40 steps vs oracle: %s"
-        % [(p["nx"], p["ny"], round(p["mach_u"], 2)) for p in params])
-    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, boundaryFirstOrder=True)
+    say("case: batch of the 3 smallest corner instances (seed 2), %s, 40 steps vs oracle: %s"
+        % (precision, [(p["nx"], p["ny"], round(p["mach_u"], 2)) for p in params]))
+    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, boundaryFirstOrder=True, precision=precision)
     meshes, W0s = build_corner_instances(params, sc)
     b = BatchSolver(meshes, sc, W0s)
     b.evolve(40)
     b.synchronize()
     dts = b.last_dt()
     for k, (m, w) in enumerate(zip(meshes, W0s)):
-        orc = O.OracleState(m.reference_arrays(sc, w), sc.R, sc.Cv, boundary_first_order=True)
+        orc = O.OracleState(m.reference_arrays(sc, w), sc.R, sc.Cv, boundary_first_order=True,
+                            mixed=precision == "mixed")
         dt = orc.run(40, sc.maxCo)
         assert dt[-1] == dts[k], k
         assert np.array_equal(b.conserved(k)[m.permutation()], orc.U), k
 
 
-def test_served_fp64_equals_oracle(afar):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_served_equals_oracle(afar, precision):
     from afar_cudaeuler2dmixo3rk3solver_amd.serve import Request, Server, build_case
-    say("case: served FP64 corner (60-step cap) and Riemann (t_end 0.01) requests vs oracle")
-    srv = Server(precision="fp64", max_active=2, chunk=20, prep_workers=1)
+    say("case: served %s corner (60-step cap) and Riemann (t_end 0.01) requests vs oracle" % precision)
+    srv = Server(precision=precision, max_active=2, chunk=20, prep_workers=1)
     reqs = [Request(0, 0.0, "corner", 4000, 21, res_tol=0.0, max_steps=60),
             Request(1, 0.0, "riemann", 3000, 22, t_end=0.01)]
     done, _ = srv.run(reqs, realtime=False)
@@ -58,7 +64,7 @@ def test_served_fp64_equals_oracle(afar):
         sc = srv.schemes_of(r)
         mesh, W0 = build_case(r, sc)
         orc = O.OracleState(mesh.reference_arrays(sc, W0), sc.R, sc.Cv,
-                            boundary_first_order=bool(sc.boundaryFirstOrder))
+                            boundary_first_order=bool(sc.boundaryFirstOrder), mixed=precision == "mixed")
         if r.kind == "corner":
             orc.run(60, sc.maxCo)
             assert r.steps == 60
@@ -88,16 +94,20 @@ def test_mixed_vortex_64_against_oracle(afar):
     mesh, W0, R, Cv = _vortex(afar, 64)
     sc64 = afar.schemes(R=R, Cv=Cv, cOrder=7, fOrder=2, maxCo=0.5)
     mesh.precompute(sc64)
-    orc = O.OracleState(mesh.reference_arrays(sc64, W0), R, Cv)
+    ref = mesh.reference_arrays(sc64, W0)
+    orc = O.OracleState(ref, R, Cv)
+    orc32 = O.OracleState(ref, R, Cv, mixed=True)  # the mixed path's own checker, on the same time steps
     gpu = afar.BasicCUDASolver(mesh, afar.schemes(R=R, Cv=Cv, cOrder=7, fOrder=2, maxCo=0.5, precision="mixed"),
                                W0)
     t = 0.0
     while t < 10.0:
         dt = min(orc.adjust_time_step(0.5), 10.0 - t)
         orc.evolve(dt)
+        orc32.evolve(dt)
         gpu.evolve(dt)
         t += dt
     U = gpu.conserved()[mesh.permutation()]
+    assert np.array_equal(U, orc32.U)
     rel = np.abs(U[:, 0] - orc.U[:, 0]).sum() / np.abs(orc.U[:, 0]).sum()
     say("  mixed vs FP64 oracle: density L1 rel %.3e" % rel)
     assert np.all(np.isfinite(U))
