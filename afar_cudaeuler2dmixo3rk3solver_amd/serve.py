@@ -122,8 +122,13 @@ class Server:
     destroys it, so that the scheduler never waits on the library's lock)."""
 
     def __init__(self, precision="mixed", hbm_budget=64 << 30, max_active=8, chunk=50, prep_workers=2,
-                 log=None, poll_s=2e-4, gpu_precompute=False, max_prepared=None):
+                 log=None, poll_s=2e-4, gpu_precompute=False, max_prepared=None, prep_order="arrival"):
+        if prep_order not in ("arrival", "smallest"):
+            raise ValueError("prep_order must be 'arrival' or 'smallest'")
         self.precision = precision
+        # which pending request a free preparation thread takes: the earliest
+        # arrival, or the one with the fewest cells (shortest job first)
+        self.prep_order = prep_order
         self.gpu_precompute = gpu_precompute
         self.budget = hbm_budget
         self.max_active = max_active
@@ -215,10 +220,11 @@ class Server:
 
     def run(self, requests, realtime=True):
         t0 = time.perf_counter()
-        prep_q = queue.Queue()
+        prep_q = queue.PriorityQueue()  # (key, request); the end markers sort after every request
         ready_q = queue.Queue()
         failed_q = queue.Queue()
         slots = threading.Semaphore(self.max_prepared)  # prepared, not yet admitted
+        by_size = self.prep_order == "smallest"
 
         def arrivals():
             for r in requests:
@@ -226,14 +232,14 @@ class Server:
                     delay = r.arrival - (time.perf_counter() - t0)
                     if delay > 0:
                         time.sleep(delay)
-                prep_q.put(r)
-            for _ in range(self.prep_workers):
-                prep_q.put(None)
+                prep_q.put(((0, r.nc_target if by_size else r.arrival, r.rid), r))
+            for k in range(self.prep_workers):
+                prep_q.put(((1, 0, k), None))
 
         def preparer():
             try:
                 while True:
-                    r = prep_q.get()
+                    r = prep_q.get()[1]
                     if r is None:
                         return
                     slots.acquire()
@@ -405,6 +411,9 @@ def main(argv=None):
     ap.add_argument("--gpu-precompute", action="store_true",
                     help="RBF precompute on the GPU, from the preparation threads (default: on the host)")
     ap.add_argument("--prep-workers", type=int, default=2, help="host threads preparing requests")
+    ap.add_argument("--prep-order", default="arrival", choices=["arrival", "smallest"],
+                    help="which pending request a free preparation thread takes: the earliest arrival, or the "
+                         "one with the fewest cells")
     ap.add_argument("--host-threads", type=int, default=0,
                     help="threads of each request's parallel host loops (0: the CPU share / prep workers)")
     args = ap.parse_args(argv)
@@ -417,7 +426,7 @@ def main(argv=None):
     reqs = request_stream(args.rate, args.duration, args.seed, args.nc_min, args.nc_max, args.t_end, args.res_tol,
                           args.max_steps)
     srv = Server(max_active=args.max_active, chunk=args.chunk, log=lambda m: print(m, flush=True),
-                 gpu_precompute=args.gpu_precompute, prep_workers=args.prep_workers)
+                 gpu_precompute=args.gpu_precompute, prep_workers=args.prep_workers, prep_order=args.prep_order)
     done, wall = srv.run(reqs)
     ok = [r for r in done if not r.error]
     tts = np.array([r.times["finish"] - r.arrival for r in ok])
@@ -447,7 +456,7 @@ def main(argv=None):
                        gpu=pct(x.get("gpu_s") for x in rows)),
         corner_residual_rel_median=float(np.median(corner_rel)) if corner_rel else None,
         cells_total=int(sum(r.cells for r in ok)), gpu_precompute=args.gpu_precompute,
-        prep_workers=args.prep_workers, max_active=args.max_active, chunk=args.chunk,
+        prep_workers=args.prep_workers, prep_order=args.prep_order, max_active=args.max_active, chunk=args.chunk,
         converged=dict(corner=sum(r.converged for r in ok if r.kind == "corner"),
                        corner_of=sum(r.kind == "corner" for r in ok),
                        riemann=sum(r.converged for r in ok if r.kind == "riemann"),

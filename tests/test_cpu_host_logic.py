@@ -78,3 +78,42 @@ def test_failed_preparation_does_not_stall_the_loop(afar, monkeypatch):
     assert all("exceeds the limit" in r.error and not r.converged for r in done)
     row = serve.summary_row(done[0])
     assert "error" in row and "gpu_s" not in row
+
+
+def test_preparation_order(afar, monkeypatch):
+    """A free preparation thread takes the earliest pending arrival, or with
+    prep_order="smallest" the pending request with the fewest cells.  (Each
+    preparation fails here, so no GPU call is made; the first one waits until
+    every other request is pending.)"""
+    import threading
+    import time
+
+    import pytest
+
+    from afar_cudaeuler2dmixo3rk3solver_amd import serve
+
+    sizes = [5000, 1000, 4000, 2000, 3000]
+    for order, key in (("arrival", lambda k: k), ("smallest", lambda k: sizes[k])):
+        taken = []
+
+        def broken(req, sc=None, gpu=False):
+            taken.append(req.rid)
+            if len(taken) == 1:
+                time.sleep(0.3)
+            raise RuntimeError("not prepared")
+
+        monkeypatch.setattr(serve, "build_case", broken)
+        reqs = [serve.Request(k, 0.0, "corner", n, k, res_tol=1e-8, max_steps=10) for k, n in enumerate(sizes)]
+        srv = serve.Server(precision="fp64", prep_workers=1, prep_order=order)
+        out = {}
+        th = threading.Thread(target=lambda: out.setdefault("r", srv.run(reqs, realtime=False)), daemon=True)
+        th.start()
+        th.join(timeout=30)
+        assert not th.is_alive()
+        # the first pick may come before the other requests are pending; the rest follow the order
+        assert sorted(taken) == list(range(5))
+        assert taken[1:] == sorted(taken[1:], key=key), (order, taken)
+        if order == "arrival":
+            assert taken == list(range(5))
+    with pytest.raises(ValueError):
+        serve.Server(prep_order="largest")
