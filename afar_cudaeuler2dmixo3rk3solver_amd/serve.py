@@ -110,10 +110,11 @@ def footprint_bytes(mesh, precision):
 class Server:
     """Continuous-batching request loop (see the module docstring).
 
-    Host threads prepare requests: mesh generation, precompute (its RBF part
-    on the GPU with gpu_precompute) and the solver with its state on the
-    device and its step graph captured (the library's calls are safe from
-    several threads).  One scheduler thread steps the solvers: it admits
+    Host threads prepare requests, the earliest arrival first or, with
+    prep_order="smallest", the pending request with the fewest cells first:
+    mesh generation, precompute (its RBF part on the GPU with gpu_precompute)
+    and the solver with its state on the device and its step graph captured
+    (the library's calls are safe from several threads).  One scheduler thread steps the solvers: it admits
     prepared requests while their footprint fits in the HBM budget and a slot
     is free, enqueues chunks of graph-launched steps on each active solver's
     stream without waiting, and polls the streams; a request whose stream is
