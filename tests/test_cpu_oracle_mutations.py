@@ -1,7 +1,8 @@
-"""The pins of tests/test_cpu_switch_pins.py catch plausible slips in the
-oracle.  Each case copies oracle/afar_oracle.cpp, plants one mistake in the
-copy, builds it on the CPU into a temporary directory and runs every pin
-against it: at least one pin must fail.  (The golden fixture is not used:
+"""The pins of tests/test_cpu_switch_pins.py and the flux pins of
+tests/test_cpu_oracle_pins.py catch plausible slips in the oracle.  Each
+case copies oracle/afar_oracle.cpp, plants one mistake in the copy, builds
+it on the CPU into a temporary directory and runs every pin against it: at
+least one pin must fail.  (The golden fixture is not used:
 it was made by the oracle itself.)  The unmodified oracle passes every pin
 (the tests of test_cpu_switch_pins.py)."""
 import concurrent.futures
@@ -39,7 +40,31 @@ MUTATIONS = [
      "g[0] = w[0]; g[1] = 0.0; g[2] = 0.0; g[3] = w[3];"),
     ("bvd_uses_raw_limiter", "const double l = to_bj(lim[3]);", "const double l = lim[3];"),
     ("cfl_boundary_velocity_dropped", "fraction * fabs(Wn[1] * Sfx + Wn[2] * Sfy)", "fraction * fabs(Wn[1] * Sfx)"),
+    # slips in the face fluxes (caught by the flux pins of test_cpu_oracle_pins.py)
+    ("roe_r2_sign", "const double r2 = deltaRho - deltaP / (cTilde * cTilde);",
+     "const double r2 = deltaRho + deltaP / (cTilde * cTilde);"),
+    ("roe_acoustic_entropy_fix_dropped",
+     "lambda1 = (lambda1 >= eps) * lambda1 + (lambda1 < eps) * ((lambda1 * lambda1 + eps * eps) / (2.0 * eps + 1e-30));",
+     "lambda1 = lambda1;"),
+    ("kfvs_internal_energy_dof", "const double K = 2.0 * Cv / R - 2.0;", "const double K = 2.0 * Cv / R;"),
+    ("central_energy_pressure_dropped",
+     "flux4[3] = 0.5 * rhoFlux * (Cv * TLeft + 0.5 * (UxLeft * UxLeft + UyLeft * UyLeft) + pLeft / rhoLeft) +",
+     "flux4[3] = 0.5 * rhoFlux * (Cv * TLeft + 0.5 * (UxLeft * UxLeft + UyLeft * UyLeft)) +"),
+    ("roe_sound_speed_gamma", "const double gamma = (Cv + R) / Cv;", "const double gamma = Cv / R;"),
 ]
+
+
+def _flux_pins():
+    from tests import test_cpu_oracle_pins as F
+    normals = [(1.0, 0.0), (0.6, -0.8), (-0.3, 0.95)]
+
+    def consistency():
+        for st in F.STATES:
+            for n in normals:
+                F.test_flux_consistency(st, n)
+
+    return [consistency, F.test_roe_upwind_supersonic, F.test_kfvs_hypersonic_upwind,
+            F.test_roe_entropy_fix_transonic_rarefaction, F.test_rk3_amplification_polynomial]
 
 
 def _build(tmp, name, old, new):
@@ -69,7 +94,7 @@ def test_pins_catch_mutation(afar, mutated_libs, monkeypatch, name):
     from tests import test_cpu_switch_pins as P
     monkeypatch.setattr(O, "_lib", O.load(mutated_libs[name]))
     failed = []
-    for pin in P.PINS:
+    for pin in P.PINS + _flux_pins():
         try:
             pin()
         except AssertionError:
