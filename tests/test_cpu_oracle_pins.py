@@ -171,7 +171,7 @@ def test_kfvs_hypersonic_upwind():
 
 
 def test_roe_entropy_fix_transonic_rarefaction():
-    """The entropy fix of the Roe flux (cudaRiemannSolver.h:113-121) on the
+    """The entropy fix of the Roe flux (cudaRiemannSolver.h:120-130) on the
     problem that needs it: Toro's test 1 (left 1, 0.75, 1; right 0.125, 0,
     0.1 as rho, u, p) has a left rarefaction through the sonic point u = c.
     A first-order finite-volume solve with the oracle's Roe flux on 200 cells
