@@ -112,7 +112,7 @@ def kernel_bytes(st, w=8, ngp=0):
                 flux_dt=flux + (64 - geo) * nf + 8 * nblk, dt_reduce=8 * nblk)
 
 
-def survey_balg_per_cell_step(st, w=8):
+def compulsory_bytes_per_cell_step(st, w=8):
     """Compulsory traffic per cell-step of the whole pipeline with no
     intermediate arrays (static data streamed once per stage)."""
     nc, nf, nb, sS, sC, sR = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"], st["sum_int_coef"]
@@ -456,7 +456,7 @@ def main():
     # pipeline's compulsory traffic with no intermediate arrays (static data
     # streamed once per stage, state read and written), evaluated on this
     # mesh's stencil and compact sizes, times the cells.
-    balg = survey_balg_per_cell_step(st, w)
+    balg = compulsory_bytes_per_cell_step(st, w)
     step_s = seconds / args.steps
     achieved = balg * mesh.nc / step_s / 1e9
     step_bytes = None
@@ -577,7 +577,7 @@ def run_batch(args):
     # The roofline unit is one RK3 step of every mesh: the pipeline's
     # compulsory bytes (no intermediate arrays) of each mesh, summed.
     w = 8 if precision == "fp64" else 4
-    step_bytes = sum(survey_balg_per_cell_step(mesh_stats(m), w) * m.nc for m in meshes)
+    step_bytes = sum(compulsory_bytes_per_cell_step(mesh_stats(m), w) * m.nc for m in meshes)
     step_s = wall / args.steps
     achieved = step_bytes / step_s / 1e9
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=PEAK_HBM_GBPS, unit="GB/s",
