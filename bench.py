@@ -573,6 +573,8 @@ def run_batch(args):
     def physical(U):
         return bool(np.isfinite(U).all() and (U[:, 0] > 0).all())
     n_physical = sum(physical(b.conserved(k)) for _, b in bb.batches for k in range(len(b.meshes)))
+    k_small = int(np.argmin([m.nc for m in meshes]))  # its state is checked against the oracle below
+    U_small = bb.conserved(k_small)[meshes[k_small].permutation()]
     bb.close()
     # The roofline unit is one RK3 step of every mesh: the pipeline's
     # compulsory bytes (no intermediate arrays) of each mesh, summed.
@@ -586,12 +588,23 @@ def run_batch(args):
                     alg_bytes_per_step=int(step_bytes), alg_bytes_per_cell_step=round(step_bytes / cells, 1),
                     avg_step_ms=round(step_s * 1e3, 5),
                     traffic_note="no counter summary for the batch: HBM bytes per step not measured")
-    cpu = None
+    cpu, accuracy = None, None
     if not args.no_cpu:
         # the oracle on the median-sized mesh of the batch, from its starting state
         k = int(np.argsort([m.nc for m in meshes])[len(meshes) // 2])
         cpu, _ = cpu_baseline(meshes[k], A.schemes(**dict(sc_kwargs, precision="fp64")), W0s[k])
         cpu["sample"] = "mesh %d of the batch (median size): " % k + cpu["sample"]
+        # the smallest mesh's state after the timed steps against the oracle of the batch's precision
+        from oracle import pyoracle as O
+        sc = A.schemes(**sc_kwargs)
+        orc = O.OracleState(meshes[k_small].reference_arrays(sc, W0s[k_small]), sc.R, sc.Cv,
+                            boundary_first_order=True, mixed=precision == "mixed")
+        orc.run(args.steps, sc.maxCo)
+        rel = np.abs(U_small - orc.U).sum(axis=0) / np.abs(orc.U).sum(axis=0)
+        accuracy = dict(reference="%s oracle (CPU), mesh %d of the batch (the smallest, %d cells), same starting "
+                                  "state and steps" % ("FP32-mixed" if precision == "mixed" else "FP64", k_small,
+                                                       meshes[k_small].nc),
+                        steps=args.steps, rho_L1_rel=float(rel[0]), bitwise_equal=bool(np.array_equal(U_small, orc.U)))
     out = {
         "metric": "Mcell-RK3-steps/s", "value": round(value, 2), "unit": "Mcell-RK3-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4),
@@ -604,7 +617,7 @@ def run_batch(args):
                    "buckets": {str(k): v for k, v in bb.tuning.items()}, "boundary_first_order": True,
                    "meshes_finite_positive_after_timed_steps": n_physical,
                    "timing": "host wall clock around the enqueue of all buckets' steps and their completion"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "accuracy": accuracy,
     }
     out["command"] = " ".join(["python", "bench.py"] + sys.argv[1:])
     print(json.dumps(out))
