@@ -44,6 +44,11 @@ struct DevPtrs {
     const int32_t* slot_face;  // sliced ELL, +-(face+1)
     const typename VT<R>::v2* slot_d;  // sliced ELL, face centre - cell centre, in the kernels' precision
     const double2* slot_e;     // sliced ELL, compact neighbour - cell centre (K3 only)
+    // FP64 four-lane reconstruction: the slot geometry and least-squares
+    // weights are formed from these instead of read from slot_d, slot_dr, lsq
+    const double2* cen;        // nc+nb cell and pseudo-cell centres (FP64; else null)
+    const double2* fcen;       // nf+nb face centres (FP64; else null)
+    const double2* fshift;     // nf periodic shift of the neighbour centre, or null when every shift is zero
     R* slot_dr;                // sliced ELL, |slot_e| in the kernels' precision (written by K3)
     typename VT<R>::v2* lsq;   // sliced ELL, least-squares weights
     const R* rbf;              // sliced ELL, RBF coefficients (face slot, stencil member)

@@ -424,6 +424,12 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
     constexpr int NT = 4 * TILE;
+    // FP64 at the face centre: lane k forms its slot's geometry (face offset,
+    // neighbour offset and distance) and least-squares weight from the staged
+    // cell centres and the face centre, with k_lsq's and the host layout's
+    // operations in their order, instead of reading 40 bytes of stored slot
+    // data per slot; the weights are shared within the group by DPP.
+    constexpr bool GEO = sizeof(R) == 8 && !GP;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     v4* sW = reinterpret_cast<v4*>(smem_raw);
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -458,12 +464,25 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     v2 Mk = v2{(R)0, (R)0};
     v2 d = v2{(R)0, (R)0};
     if (inb && k < C) {
-        Mk = p.lsq[sell(c, k, maxC)];
         sf = p.slot_face[sell(c, k, maxC)];
-        d = p.slot_d[sell(c, k, maxC)];
+        if constexpr (!GEO) {
+            Mk = p.lsq[sell(c, k, maxC)];
+            d = p.slot_d[sell(c, k, maxC)];
+        }
     }
     v4 wHalo;
     if (t < hn) wHalo = p.W[h0];
+    double2 cOwn = double2{0.0, 0.0}, cHalo = double2{0.0, 0.0};  // GEO: staged centres
+    double2 fc = double2{0.0, 0.0}, fs = double2{0.0, 0.0};        // GEO: face centre, periodic shift
+    if constexpr (GEO) {
+        if (ownW) cOwn = p.cen[tile * TILE + t];
+        if (t < hn) cHalo = p.cen[h0];
+        if (inb && k < C) {
+            const int ff = (sf < 0 ? -sf : sf) - 1;
+            fc = p.fcen[ff];
+            if (p.fshift && ff < nf) fs = p.fshift[ff];
+        }
+    }
     // A cell that took the full path at the previous launch most likely takes
     // it again: its RBF coefficients and centre distance are requested in this
     // round already (flat_prev only schedules loads; results do not depend on it).
@@ -472,18 +491,25 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     const R* a = p.rbf + sell(c, k * (maxS + 1), KR);  // row stride 64
     const bool early = inb && k < C && !wasFlat;
     if (early) {
-        deltaR = p.slot_dr[sell(c, k, maxC)];
+        if constexpr (!GEO) deltaR = p.slot_dr[sell(c, k, maxC)];
 #pragma unroll
         for (int i = 0; i <= kMaxSten; ++i)
             if (i <= S) cf[i] = a[(size_t)i * 64];
     }
     uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_sten_off);
     v2* sM = reinterpret_cast<v2*>(smem_raw + p.lds_m_off);  // 4 weights per cell
+    double2* sC = reinterpret_cast<double2*>(smem_raw + p.lds_m_off);  // GEO: tile + halo centres instead
     uint4* stDst = reinterpret_cast<uint4*>(sSt);
     if (ownW) sW[t] = wOwn;
     if (t < hn) sW[TILE + t] = wHalo;
     if (t < n16) stDst[t] = st0;
-    sM[t] = Mk;
+    if constexpr (GEO) {
+        if (ownW) sC[t] = cOwn;
+        if (t < hn) sC[TILE + t] = cHalo;
+        for (int i = t + NT; i < hn; i += NT) sC[TILE + i] = p.cen[p.halo[hb + i]];
+    } else {
+        sM[t] = Mk;
+    }
     for (int i = t + NT; i < hn; i += NT) sW[TILE + i] = p.W[p.halo[hb + i]];  // halos beyond NT cells
     for (int i = t + NT; i < n16; i += NT) stDst[i] = stSrc[i];
     __syncthreads();
@@ -502,36 +528,90 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
     // extrema are exact) and stores the gradient; the group then shares them.
     // The full path does this after its RBF sums, so that fewer values are
     // live while the coefficients are in registers.
+    const int isNei = sf < 0;
+    const int f = (sf < 0 ? -sf : sf) - 1;
+    const bool live = k < C && f < nf;
+    double ex = 0.0, ey = 0.0;  // GEO: compact neighbour - cell centre of lane k's slot
+    if constexpr (GEO) {
+        // the host layout's expressions (face offset d, neighbour offset e)
+        if (k < C) {
+            const double2 cc = sC[lc];
+            const double2 cn = sC[sSt[sell(lc, k, maxS)]];
+            if (f < nf && !isNei) {
+                d = v2{fc.x - cc.x, fc.y - cc.y};
+                ex = (cn.x + fs.x) - cc.x;
+                ey = (cn.y + fs.y) - cc.y;
+            } else if (f < nf) {
+                const double xn = cc.x + fs.x, yn = cc.y + fs.y;
+                d = v2{fc.x - xn, fc.y - yn};
+                ex = cn.x - xn;
+                ey = cn.y - yn;
+            } else {
+                d = v2{fc.x - cc.x, fc.y - cc.y};
+                ex = cn.x - cc.x;
+                ey = cn.y - cc.y;
+            }
+        }
+        // k_lsq's sums over the compact slots in order, then lane k's weight
+        double Sxx = 0.0, Sxy = 0.0, Syy = 0.0;
+        const double bx[4] = {quad_bcast<0>(ex), quad_bcast<1>(ex), quad_bcast<2>(ex), quad_bcast<3>(ex)};
+        const double by[4] = {quad_bcast<0>(ey), quad_bcast<1>(ey), quad_bcast<2>(ey), quad_bcast<3>(ey)};
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (i < C) {
+                Sxx += bx[i] * bx[i];
+                Sxy += bx[i] * by[i];
+                Syy += by[i] * by[i];
+            }
+        if (k < C)
+            Mk = v2{(Syy * ex - Sxy * ey) / (Sxx * Syy - Sxy * Sxy), (-Sxy * ex + Sxx * ey) / (Sxx * Syy - Sxy * Sxy)};
+    }
     R gxk = 0, gyk = 0, mxk = 0, mnk = 0;
     auto gradient = [&]() {
         const R wck = sWr[4 * lc + k];
         mxk = wck;
         mnk = wck;
-        for (int i = 0; i < C; ++i) {
-            const v2 M = sM[4 * lc + i];
-            const R cj = sWr[4 * sSt[sell(lc, i, maxS)] + k];
-            const R dphi = cj - wck;
-            gxk += M.x * dphi;
-            gyk += M.y * dphi;
-            mxk = fmax(cj, mxk);
-            mnk = fmin(cj, mnk);
+        if constexpr (GEO) {
+            // every slot's weight from its lane, in slot order
+            const double mx[4] = {quad_bcast<0>(Mk.x), quad_bcast<1>(Mk.x), quad_bcast<2>(Mk.x), quad_bcast<3>(Mk.x)};
+            const double my[4] = {quad_bcast<0>(Mk.y), quad_bcast<1>(Mk.y), quad_bcast<2>(Mk.y), quad_bcast<3>(Mk.y)};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (i < C) {
+                    const R cj = sWr[4 * sSt[sell(lc, i, maxS)] + k];
+                    const R dphi = cj - wck;
+                    gxk += mx[i] * dphi;
+                    gyk += my[i] * dphi;
+                    mxk = fmax(cj, mxk);
+                    mnk = fmin(cj, mnk);
+                }
+        } else {
+            for (int i = 0; i < C; ++i) {
+                const v2 M = sM[4 * lc + i];
+                const R cj = sWr[4 * sSt[sell(lc, i, maxS)] + k];
+                const R dphi = cj - wck;
+                gxk += M.x * dphi;
+                gyk += M.y * dphi;
+                mxk = fmax(cj, mxk);
+                mnk = fmin(cj, mnk);
+            }
         }
         v2* g = k == 0 ? p.gp : k == 1 ? p.gu : k == 2 ? p.gv : p.gT;
         g[c] = v2{gxk, gyk};
     };
-    const int isNei = sf < 0;
-    const int f = (sf < 0 ? -sf : sf) - 1;
-    const bool live = k < C && f < nf;
     const bool flat = fabs(TMax - TMin) < K<R>::VSMALL;
 
     if (k == 0) p.flat_prev[c] = flat;
     // Full path not foreseen: every RBF coefficient of the face and its centre
     // distance are requested now, at once (one memory round trip).
     if (!flat && live && !early) {
-        deltaR = p.slot_dr[sell(c, k, maxC)];
+        if constexpr (!GEO) deltaR = p.slot_dr[sell(c, k, maxC)];
 #pragma unroll
         for (int i = 0; i <= kMaxSten; ++i)
             if (i <= S) cf[i] = a[(size_t)i * 64];
+    }
+    if constexpr (GEO) {
+        if (!flat && live) deltaR = afar_hypot(ex, ey);  // k_lsq's centre distance
     }
     const R dx = d.x, dy = d.y;
     R fraction = 0;

@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <mutex>
@@ -102,7 +103,10 @@ class SolverImpl final : public DeviceSolver {
         else
             lanes_ = L_.maxC > 4 ? 1 : 4;  // four lanes measured fastest at 64 and 128-cell tiles
         // the four-lane kernel also stages the tile's least-squares weights
-        if (lanes_ == 4) lds_recon_ = lds_m_off_ + (size_t)L_.tile * 4 * sizeof(v2);
+        // (FP64 at the face centre: the tile's and halo's centres in their place)
+        if (lanes_ == 4)
+            lds_recon_ = lds_m_off_ + std::max((size_t)L_.tile * 4 * sizeof(v2),
+                                               sizeof(R) == 8 ? (size_t)(L_.tile + L_.maxHalo) * 16 : (size_t)0);
         // BVD: the tile's and halo's smoothness flags, then the stencil ids
         lds_bvd_sten_off_ = ((size_t)(L_.tile + L_.maxHalo) + 15) & ~(size_t)15;
         lds_bvd_ = lds_bvd_sten_off_ + (size_t)L_.tile * L_.maxS * sizeof(uint16_t);
@@ -666,6 +670,21 @@ class SolverImpl final : public DeviceSolver {
         }
         P_.fcell = reinterpret_cast<const int2*>(put(m_fcell, L_.fcell.data(), 2 * nf));
         P_.fgeo = reinterpret_cast<const double4*>(put(m_fgeo, L_.fgeo.data(), 8 * nf));
+        // FP64: cell and face centres (and periodic shifts) for the four-lane
+        // reconstruction, which forms its slot geometry from them
+        P_.cen = P_.fcen = P_.fshift = nullptr;
+        if (sizeof(R) == 8) {
+            P_.cen = reinterpret_cast<const double2*>(put(m_cen, ra_.CELL.data(), 2 * (nc + nb)));
+            std::vector<double> fc(2 * (nf + nb));
+            for (size_t i = 0; i < nf + nb; ++i) {
+                fc[2 * i] = ra_.FACE[5 * i];
+                fc[2 * i + 1] = ra_.FACE[5 * i + 1];
+            }
+            P_.fcen = reinterpret_cast<const double2*>(put(m_fcen, fc.data(), fc.size()));
+            bool shifted = false;
+            for (double v : ra_.FSHIFT) shifted = shifted || v != 0.0 || std::signbit(v);  // -0.0 counts
+            if (shifted) P_.fshift = reinterpret_cast<const double2*>(put(m_fshift, ra_.FSHIFT.data(), 2 * nf));
+        }
         P_.fgeo32 = nullptr;
         if (sizeof(R) == 4) {
             std::vector<float> g(L_.fgeo.size());
@@ -779,7 +798,7 @@ class SolverImpl final : public DeviceSolver {
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_slot_dr, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fgeo32,
         m_fv, m_fvp, m_fvu, m_fvv, m_flux, m_flux32, m_bcell, m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res,
-        m_rbf_gp, m_fgd, m_fvg, m_flat_prev, m_tlim, m_tTVD;
+        m_rbf_gp, m_fgd, m_fvg, m_flat_prev, m_tlim, m_tTVD, m_cen, m_fcen, m_fshift;
 };
 
 // A solver on relabelled cells and faces (compact_tile_order) that reads and
