@@ -68,29 +68,33 @@ def mesh_stats(mesh, field=None, precision="fp64"):
         Tm, Ts = Tm.astype(ft), Ts.astype(ft)
         rng = np.maximum(Ts.max(axis=1), Tm[:nc]) - np.minimum(Ts.min(axis=1), Tm[:nc])
         full = ~(np.abs(rng) < ft(1e-10))
-    return dict(nc=nc, nf=nf, nb=s["nb"], sumS=int(S.sum()), sumC=int(Cc.sum()),
+    return dict(nc=nc, nf=nf, nb=s["nb"], maxC=s["maxC"], sumS=int(S.sum()), sumC=int(Cc.sum()),
                 sum_int_coef=int(coef.sum()), sum_int_coef_full=int(coef[full].sum()),
                 int_slots_full=int(n_int[full].sum()), full_cells=int(full.sum()),
                 meanS=float(S.mean()), meanC=float(Cc.mean()))
 
 
-def kernel_bytes(st, w=8, ngp=0):
+def kernel_bytes(st, w=8, ngp=0, geo=False):
     """Compulsory bytes per launch of each kernel (every array element touched
     once; neighbour re-reads counted as cache hits).  w = width of the
     reconstruction/flux type (8 FP64, 4 FP32-mixed); state stays FP64.  The
     reconstruction reads the RBF coefficients and the neighbour-centre
     distances (slot_dr) only for cells on the full path; face offsets
-    (slot_d) and distances are stored in the kernels' precision.  ngp > 0 (flux quadrature
-    points): the reconstruction also reads each point's coefficients (full
-    path) and offsets and writes each point's states; the flux reads those
-    states and offsets instead of the face-centre states."""
+    (slot_d) and distances are stored in the kernels' precision.  geo (the
+    FP64 four-lane kernel at the face centre): instead of the face offsets,
+    distances and least-squares weights it reads the cell and pseudo-cell
+    centres and, per compact slot, the face centre (16 B each).  ngp > 0
+    (flux quadrature points): the reconstruction also reads each point's
+    coefficients (full path) and offsets and writes each point's states; the
+    flux reads those states and offsets instead of the face-centre states."""
     nc, nf, nb, sS, sC = st["nc"], st["nf"], st["nb"], st["sumS"], st["sumC"]
     sR, sfull = st["sum_int_coef_full"], st["int_slots_full"]
     ns = 2 * nf  # internal face slots
+    slot_geo = 16 * (nc + nb) + 16 * sC if geo else 2 * w * ns + w * sfull + 2 * w * sC
     # writes: gradients 8w, limiters 4w, smoothness flag and limiter condition per cell; O3 face state 4w
     # (FP64: one array per variable, the temperatures being what the BVD test reads; FP32: records and the
     # temperatures again, w) and the BVD test's limited linear temperature w per internal face side
-    recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + 2 * w * ns + w * sfull + 2 * w * sC + w * sR
+    recon = (4 * w * (nc + nb) + 2 * nc + 2 * sS + 4 * sC + slot_geo + w * sR
              + 8 * w * nc + 4 * w * nc + 2 * nc + 4 * w * ns + w * ns + (w * ns if w == 4 else 0))
     # bvd reads flags, counts, stencil ids, face slots, both sides' O3 face temperatures and this side's
     # limited linear one, the limiter condition; writes the types (the face pass applies the limiter transform)
@@ -438,7 +442,8 @@ def main():
         same = same and bool(np.array_equal(gpu.conserved(), U_t1))
     w = 8 if args.precision == "fp64" else 4
     ngp = 2 if args.face_quadrature else 0
-    kb = kernel_bytes(st, w, ngp)
+    lanes = args.recon_lanes or (4 if st["maxC"] <= 4 else 1)
+    kb = kernel_bytes(st, w, ngp, geo=w == 8 and ngp == 0 and lanes == 4)
     kernels = {}
     for k, (avg_b2b, cnt) in b2b.items():
         avg = live[k][0] if live[k][1] else avg_b2b
