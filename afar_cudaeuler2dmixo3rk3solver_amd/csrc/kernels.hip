@@ -978,23 +978,29 @@ __device__ __forceinline__ void roe_flux(R& rhoFlux, R& rhoUxFlux, R& rhoUyFlux,
 
 // Waves per SIMD the register allocator must allow for the face-centre flux
 // pass, per working type (build-time tuning knobs; 1 = no constraint).  The
-// FP64 pass takes 98-100 VGPRs (5 waves) unconstrained; 6 and 8 waves spill
-// (36 and 132 bytes per lane) and measured 26% and 108% slower per launch.
+// FP64 passes take 104 VGPRs (4 waves) unconstrained.  At 5 waves the pass
+// without the CFL bound fits in 92 VGPRs without spills, yet measured 4%
+// slower per launch (the pass with it spills 20 bytes per lane); 6 and 8
+// waves spill (36 and 132 bytes per lane) and measured 26% and 108% slower.
 #ifndef AFAR_FLUX_WPE_D
 #define AFAR_FLUX_WPE_D 1
+#endif
+#ifndef AFAR_FLUX_DT_WPE_D
+#define AFAR_FLUX_DT_WPE_D 1
 #endif
 #ifndef AFAR_FLUX_WPE_F
 #define AFAR_FLUX_WPE_F 1
 #endif
-template <typename R, bool GP> struct FluxWaves {
-    static constexpr int value = GP ? 1 : (sizeof(R) == 8 ? AFAR_FLUX_WPE_D : AFAR_FLUX_WPE_F);
+template <typename R, bool DT, bool GP> struct FluxWaves {
+    static constexpr int value =
+        GP ? 1 : (sizeof(R) == 8 ? (DT ? AFAR_FLUX_DT_WPE_D : AFAR_FLUX_WPE_D) : AFAR_FLUX_WPE_F);
 };
 
 // DT: also reduce the CFL bound of every internal face to one partial minimum
 // per block (dt_part[blockIdx.x]); the bound is a function of the
 // primitive state alone, which no kernel of the stage has changed yet.
 template <typename R, bool DT, bool GP>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R, GP>::value, 8))) void k_flux(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R, DT, GP>::value, 8))) void k_flux(
     DevPtrs<R> p, double Rgas, double Cv, double CFL) {
     using v4 = typename VT<R>::v4;
     using v2 = typename VT<R>::v2;
@@ -1011,6 +1017,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R
             const float4 h0 = p.fgeo32[2 * f], h1 = p.fgeo32[2 * f + 1];
             g0 = double4{h0.x, h0.y, h0.z, h0.w};
             g1 = double4{h1.x, h1.y, h1.z, h1.w};
+        } else if constexpr (sizeof(R) == 8 && !GP) {
+            // the host layout's expressions: offsets of the face centre from
+            // the owner's centre and from the (shifted) neighbour's centre,
+            // and the distance between the two centres
+            const double2 sf = p.fsf[f], fc = p.fcen[f], co = p.cen[own], cn = p.cen[nei];
+            const double2 fs = p.fshift ? p.fshift[f] : double2{0.0, 0.0};
+            const double xn = cn.x + fs.x, yn = cn.y + fs.y;
+            double dR = 0.0;
+            if (DT) {
+                const double ex = xn - co.x, ey = yn - co.y;
+                dR = afar_hypot(ex, ey);
+            }
+            g0 = double4{sf.x, sf.y, p.fmag[f], dR};
+            g1 = double4{fc.x - co.x, fc.y - co.y, fc.x - xn, fc.y - yn};
         } else {
             g0 = p.fgeo[2 * f];
             g1 = p.fgeo[2 * f + 1];

@@ -685,6 +685,20 @@ class SolverImpl final : public DeviceSolver {
             for (double v : ra_.FSHIFT) shifted = shifted || v != 0.0 || std::signbit(v);  // -0.0 counts
             if (shifted) P_.fshift = reinterpret_cast<const double2*>(put(m_fshift, ra_.FSHIFT.data(), 2 * nf));
         }
+        // FP64: the face-centre flux forms its offsets from the centres above
+        // and reads Sf and magSf alone (40 instead of 64 B per face)
+        P_.fsf = nullptr;
+        P_.fmag = nullptr;
+        if (sizeof(R) == 8) {
+            std::vector<double> sf(2 * (size_t)nf), mag(nf);
+            for (size_t i = 0; i < nf; ++i) {
+                mag[i] = ra_.FACE[5 * i + 2];
+                sf[2 * i] = ra_.FACE[5 * i + 3];
+                sf[2 * i + 1] = ra_.FACE[5 * i + 4];
+            }
+            P_.fsf = reinterpret_cast<const double2*>(put(m_fsf, sf.data(), sf.size()));
+            P_.fmag = put(m_fmag, mag.data(), mag.size());
+        }
         P_.fgeo32 = nullptr;
         if (sizeof(R) == 4) {
             std::vector<float> g(L_.fgeo.size());
@@ -798,7 +812,7 @@ class SolverImpl final : public DeviceSolver {
     DevMem m_W, m_U, m_Uold, m_vol, m_gp, m_gu, m_gv, m_gT, m_lim, m_smooth, m_ftype, m_scnt, m_ccnt, m_sten,
         m_slot_face, m_slot_d, m_slot_e, m_slot_dr, m_lsq, m_rbf, m_halo_ptr, m_halo, m_fcell, m_fgeo, m_fgeo32,
         m_fv, m_fvp, m_fvu, m_fvv, m_flux, m_flux32, m_bcell, m_blabel, m_bgeo, m_bval, m_dt, m_fvT, m_cinst, m_finst, m_tctl, m_res,
-        m_rbf_gp, m_fgd, m_fvg, m_flat_prev, m_tlim, m_tTVD, m_cen, m_fcen, m_fshift;
+        m_rbf_gp, m_fgd, m_fvg, m_flat_prev, m_tlim, m_tTVD, m_cen, m_fcen, m_fshift, m_fsf, m_fmag;
 };
 
 // A solver on relabelled cells and faces (compact_tile_order) that reads and

@@ -100,10 +100,15 @@ def kernel_bytes(st, w=8, ngp=0, geo=False):
     # limited linear one, the limiter condition; writes the types (the face pass applies the limiter transform)
     bvd = ((nc + nb) + 2 * nc + 2 * sS + 4 * sC + w * ns + w * ns + nc + nc)
     # FP32-mixed: internal fluxes at the face centre are stored as floats, and
-    # the flux pass without the CFL bound reads the face geometry in float
+    # the flux pass without the CFL bound reads the face geometry in float.
+    # FP64 at the face centre: both passes read Sf, magSf and the face centre
+    # (40 B per face) and the cells' centres (16 B per cell; a periodic mesh
+    # adds its 16-B shift per face), and form the offsets and the distance
     fw = 4 * w if (w == 4 and not ngp) else 32
-    geo = 8 * w
-    flux = 8 * nf + geo * nf + 8 * w * nf + nc * (4 * w + 4 * w + 8 * w + 1) + 32 * nb + 4 * w * nb + fw * nf + 32 * nb
+    cg = w == 8 and not ngp
+    geo = 40 if cg else 8 * w
+    flux = (8 * nf + geo * nf + 8 * w * nf + nc * (4 * w + 4 * w + 8 * w + 1 + (16 if cg else 0)) + 32 * nb
+            + 4 * w * nb + fw * nf + 32 * nb)
     # U and Uold are read in the second and third stages, Uold alone in the first; the first two stages
     # write U, the last writes Uold only
     update = nc + 4 * sC + fw * nf + 32 * nb + 32 * nc * 5 / 3 + 8 * nc + 65 * nb + 32 * nc + 4 * w * (nc + nb)
@@ -113,7 +118,7 @@ def kernel_bytes(st, w=8, ngp=0, geo=False):
         flux += ngp * (8 * w + 4 * w) * nf - 8 * w * nf
     nblk = 8 * (((nf + nb + 255) // 256 + 7) // 8)  # flux grid: one CFL partial per block
     return dict(cfl=cfl, reconstruct=recon, bvd=bvd, flux=flux, update=update,
-                flux_dt=flux + (64 - geo) * nf + 8 * nblk, dt_reduce=8 * nblk)
+                flux_dt=flux + (0 if cg else (64 - geo) * nf) + 8 * nblk, dt_reduce=8 * nblk)
 
 
 def compulsory_bytes_per_cell_step(st, w=8):
