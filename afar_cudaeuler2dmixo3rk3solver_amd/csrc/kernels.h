@@ -46,14 +46,15 @@ struct DevPtrs {
     const double2* slot_e;     // sliced ELL, compact neighbour - cell centre (K3 only)
     // FP64 four-lane reconstruction: the slot geometry and least-squares
     // weights are formed from these instead of read from slot_d, slot_dr, lsq
-    const double2* cen;        // nc+nb cell and pseudo-cell centres (FP64; else null)
-    const double2* fcen;       // nf+nb face centres (FP64; else null)
+    const double2* cen;        // nc+nb cell and pseudo-cell centres
+    const double2* fcen;       // nf+nb face centres
     const double2* fshift;     // nf periodic shift of the neighbour centre, or null when every shift is zero
-    // FP64 face-centre flux: the face's offsets from both cell centres (and
-    // the centre distance of the CFL bound) are formed from cen, fcen, fshift;
+    // Face-centre flux passes with the FP64 geometry (FP64; FP32-mixed with
+    // the CFL bound): the face's offsets from both cell centres (and the
+    // centre distance of the CFL bound) are formed from cen, fcen, fshift;
     // the face area vector and its magnitude are read from these
-    const double2* fsf;        // nf Sf (FP64; else null)
-    const double* fmag;        // nf magSf (FP64; else null)
+    const double2* fsf;        // nf Sf
+    const double* fmag;        // nf magSf
     R* slot_dr;                // sliced ELL, |slot_e| in the kernels' precision (written by K3)
     typename VT<R>::v2* lsq;   // sliced ELL, least-squares weights
     const R* rbf;              // sliced ELL, RBF coefficients (face slot, stencil member)

@@ -1017,10 +1017,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R
             const float4 h0 = p.fgeo32[2 * f], h1 = p.fgeo32[2 * f + 1];
             g0 = double4{h0.x, h0.y, h0.z, h0.w};
             g1 = double4{h1.x, h1.y, h1.z, h1.w};
-        } else if constexpr (sizeof(R) == 8 && !GP) {
-            // the host layout's expressions: offsets of the face centre from
-            // the owner's centre and from the (shifted) neighbour's centre,
-            // and the distance between the two centres
+        } else if constexpr (!GP) {
+            // the host layout's expressions in FP64: offsets of the face
+            // centre from the owner's centre and from the (shifted)
+            // neighbour's centre, and the distance between the two centres
+            // (FP32-mixed rounds the offsets to float below, as it rounds the
+            // stored record)
             const double2 sf = p.fsf[f], fc = p.fcen[f], co = p.cen[own], cn = p.cen[nei];
             const double2 fs = p.fshift ? p.fshift[f] : double2{0.0, 0.0};
             const double xn = cn.x + fs.x, yn = cn.y + fs.y;

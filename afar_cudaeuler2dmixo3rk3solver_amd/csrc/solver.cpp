@@ -670,10 +670,11 @@ class SolverImpl final : public DeviceSolver {
         }
         P_.fcell = reinterpret_cast<const int2*>(put(m_fcell, L_.fcell.data(), 2 * nf));
         P_.fgeo = reinterpret_cast<const double4*>(put(m_fgeo, L_.fgeo.data(), 8 * nf));
-        // FP64: cell and face centres (and periodic shifts) for the four-lane
-        // reconstruction, which forms its slot geometry from them
+        // Cell and face centres (and periodic shifts): the FP64 four-lane
+        // reconstruction forms its slot geometry from them, and the
+        // face-centre flux passes their offsets and the CFL bound's distance
         P_.cen = P_.fcen = P_.fshift = nullptr;
-        if (sizeof(R) == 8) {
+        {
             P_.cen = reinterpret_cast<const double2*>(put(m_cen, ra_.CELL.data(), 2 * (nc + nb)));
             std::vector<double> fc(2 * (nf + nb));
             for (size_t i = 0; i < nf + nb; ++i) {
@@ -685,11 +686,13 @@ class SolverImpl final : public DeviceSolver {
             for (double v : ra_.FSHIFT) shifted = shifted || v != 0.0 || std::signbit(v);  // -0.0 counts
             if (shifted) P_.fshift = reinterpret_cast<const double2*>(put(m_fshift, ra_.FSHIFT.data(), 2 * nf));
         }
-        // FP64: the face-centre flux forms its offsets from the centres above
-        // and reads Sf and magSf alone (40 instead of 64 B per face)
+        // The face-centre flux passes that use the FP64 geometry (every FP64
+        // pass, the FP32-mixed pass with the CFL bound) form their offsets
+        // from the centres above and read Sf and magSf alone (40 instead of
+        // 64 B per face)
         P_.fsf = nullptr;
         P_.fmag = nullptr;
-        if (sizeof(R) == 8) {
+        {
             std::vector<double> sf(2 * (size_t)nf), mag(nf);
             for (size_t i = 0; i < nf; ++i) {
                 mag[i] = ra_.FACE[5 * i + 2];

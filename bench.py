@@ -103,9 +103,16 @@ def kernel_bytes(st, w=8, ngp=0, geo=False):
     # the flux pass without the CFL bound reads the face geometry in float.
     # FP64 at the face centre: both passes read Sf, magSf and the face centre
     # (40 B per face) and the cells' centres (16 B per cell; a periodic mesh
-    # adds its 16-B shift per face), and form the offsets and the distance
+    # adds its 16-B shift per face), and form the offsets and the distance;
+    # so does the FP32-mixed pass with the CFL bound
     fw = 4 * w if (w == 4 and not ngp) else 32
     cg = w == 8 and not ngp
+    if cg:
+        dt_extra = 0
+    elif not ngp:
+        dt_extra = (40 - 8 * w) * nf + 16 * nc
+    else:  # flux points: the 64-B FP64 record
+        dt_extra = (64 - 8 * w) * nf
     geo = 40 if cg else 8 * w
     flux = (8 * nf + geo * nf + 8 * w * nf + nc * (4 * w + 4 * w + 8 * w + 1 + (16 if cg else 0)) + 32 * nb
             + 4 * w * nb + fw * nf + 32 * nb)
@@ -118,7 +125,7 @@ def kernel_bytes(st, w=8, ngp=0, geo=False):
         flux += ngp * (8 * w + 4 * w) * nf - 8 * w * nf
     nblk = 8 * (((nf + nb + 255) // 256 + 7) // 8)  # flux grid: one CFL partial per block
     return dict(cfl=cfl, reconstruct=recon, bvd=bvd, flux=flux, update=update,
-                flux_dt=flux + (0 if cg else (64 - geo) * nf) + 8 * nblk, dt_reduce=8 * nblk)
+                flux_dt=flux + dt_extra + 8 * nblk, dt_reduce=8 * nblk)
 
 
 def compulsory_bytes_per_cell_step(st, w=8):

@@ -191,6 +191,24 @@ def test_periodic_vortex_parity(afar, split_p):
     assert np.array_equal(gpu.conserved()[mesh.permutation()], orc.U)
 
 
+def test_periodic_vortex_device_time_step(afar):
+    """Periodic vortex with the device time step (graph replays): the faces
+    across the wrap take the shifted neighbour centre in their offsets and in
+    the CFL bound's centre distance, which the face pass forms on the device;
+    time steps and state equal the oracle's host loop."""
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    say("case: periodic vortex 24x24 split_p 0.5, 40 device-dt graph steps")
+    mesh, W0 = cases.vortex_case(n=24, seed=4, split_p=0.5)
+    sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5)
+    gpu, orc, ref = make_pair(afar, mesh, W0, sc)
+    assert np.any(ref["faceShift"] != 0.0)
+    dts = orc.run(40, sc.maxCo)
+    gpu.evolve_auto(40)
+    gpu.synchronize()
+    assert gpu.last_dt() == dts[-1]
+    assert np.array_equal(gpu.conserved()[mesh.permutation()], orc.U)
+
+
 def test_riemann_quads_to_t03_parity(afar):
     """Jittered quad mesh, four-quadrant Riemann problem run to t = 0.3."""
     from afar_cudaeuler2dmixo3rk3solver_amd import cases
