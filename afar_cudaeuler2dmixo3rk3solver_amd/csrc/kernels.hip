@@ -771,12 +771,14 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const int maxS = p.maxS, maxC = p.maxC, nf = p.nf;
     const bool inb = c < p.nc;
     // Two rounds of independent loads before the barrier: (1) the cell's
-    // counts, flag and limiter condition, the first slots' faces, halo ids
-    // and the stencil block; (2) halo flags and the slots' face temperatures
-    // (both sides' O3 values and this side's limited linear value, formed by
-    // the reconstruction).  Indices are clamped rather than branched on, and
+    // counts, flag and limiter condition, the first slots' faces and halo
+    // ids; (2) halo flags and the slots' face temperatures (both sides' O3
+    // values and this side's limited linear value, formed by the
+    // reconstruction).  Indices are clamped rather than branched on, and
     // values held in plain scalars, so that no load is waited for before its
-    // round is complete.
+    // round is complete.  The stencil block is read only by tiles with a
+    // flagged cell among their own and halo cells: elsewhere every flag sum
+    // is zero.
     const int cc = inb ? c : p.nc - 1;
     const int S = p.scnt[cc], C = inb ? (int)p.ccnt[cc] : 0;
     const int8_t own = p.smooth[cc];
@@ -789,9 +791,6 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const int nw = min(wpt, ((p.nc + 63) >> 6) - tile * wpt);
     const int n16 = nw * maxS * 8;
     const uint4* stSrc = reinterpret_cast<const uint4*>(p.sten + (size_t)tile * wpt * maxS * 64);
-    const uint4 st0 = stSrc[min(tid, n16 - 1)];
-    const uint4 st1 = stSrc[min(tid + TILE, n16 - 1)];
-    const uint4 st2 = stSrc[min(tid + 2 * TILE, n16 - 1)];
     const int hmax = hn > 0 ? hn - 1 : 0;  // a tile without halo reads its first id harmlessly
     const unsigned hid0 = (unsigned)p.halo[hb + min(tid, hmax)];
     const unsigned hid1 = (unsigned)p.halo[hb + min(tid + TILE, hmax)];
@@ -809,23 +808,37 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
         tvd[fi] = p.tTVD[2 * fs + (sf < 0)];
     }
     if (inb) sS[tid] = own;
-    uint4* stDst = reinterpret_cast<uint4*>(sSt);
     if (tid < hn) sS[TILE + tid] = hs0;
     if (tid + TILE < hn) sS[2 * TILE + tid] = hs1;
-    if (tid < n16) stDst[tid] = st0;
-    if (tid + TILE < n16) stDst[tid + TILE] = st1;
-    if (tid + 2 * TILE < n16) stDst[tid + 2 * TILE] = st2;
-    for (int i = tid + 2 * TILE; i < hn; i += TILE) sS[TILE + i] = p.smooth[p.halo[hb + i]];
-    for (int i = tid + 3 * TILE; i < n16; i += TILE) stDst[i] = stSrc[i];
-    __syncthreads();
-    if (!inb) return;
+    int flagged = (inb && own != 0) || (tid < hn && hs0 != 0) || (tid + TILE < hn && hs1 != 0);
+    for (int i = tid + 2 * TILE; i < hn; i += TILE) {
+        const int8_t h = p.smooth[p.halo[hb + i]];
+        sS[TILE + i] = h;
+        flagged |= h != 0;
+    }
     int8_t shockType = own;
     int8_t shockTypeExt = own;
-    for (int ci = 0; ci < C; ++ci) {
-        shockType += sS[sSt[sell(tid, ci, maxS)]];
-        shockTypeExt = shockType;
+    // block-uniform: one wavefront's vote, or a workgroup reduction
+    const bool anyFlagged = TILE == 64 ? __any(flagged) != 0 : __syncthreads_or(flagged) != 0;
+    if (anyFlagged) {
+        uint4* stDst = reinterpret_cast<uint4*>(sSt);
+        const uint4 st0 = stSrc[min(tid, n16 - 1)];
+        const uint4 st1 = stSrc[min(tid + TILE, n16 - 1)];
+        const uint4 st2 = stSrc[min(tid + 2 * TILE, n16 - 1)];
+        if (tid < n16) stDst[tid] = st0;
+        if (tid + TILE < n16) stDst[tid + TILE] = st1;
+        if (tid + 2 * TILE < n16) stDst[tid + 2 * TILE] = st2;
+        for (int i = tid + 3 * TILE; i < n16; i += TILE) stDst[i] = stSrc[i];
+        __syncthreads();
+        if (inb) {
+            for (int ci = 0; ci < C; ++ci) {
+                shockType += sS[sSt[sell(tid, ci, maxS)]];
+                shockTypeExt = shockType;
+            }
+            for (int ci = C; ci < S; ++ci) shockTypeExt += sS[sSt[sell(tid, ci, maxS)]];
+        }
     }
-    for (int ci = C; ci < S; ++ci) shockTypeExt += sS[sSt[sell(tid, ci, maxS)]];
+    if (!inb) return;
 
     // Every face is visited; the reference leaves the loop at a cell's second
     // boundary face with bvd = 1, the largest value, so the result is the same.
