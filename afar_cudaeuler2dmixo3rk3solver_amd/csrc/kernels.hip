@@ -772,13 +772,13 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const bool inb = c < p.nc;
     // Two rounds of independent loads before the barrier: (1) the cell's
     // counts, flag and limiter condition, the first slots' faces and halo
-    // ids; (2) halo flags and the slots' face temperatures (both sides' O3
-    // values and this side's limited linear value, formed by the
-    // reconstruction).  Indices are clamped rather than branched on, and
-    // values held in plain scalars, so that no load is waited for before its
-    // round is complete.  The stencil block is read only by tiles with a
-    // flagged cell among their own and halo cells: elsewhere every flag sum
-    // is zero.
+    // ids; (2) halo flags and, in cells meeting the limiter condition, the
+    // slots' face temperatures (both sides' O3 values and this side's limited
+    // linear value, formed by the reconstruction).  Halo and stencil indices
+    // are clamped rather than branched on, and values held in plain scalars,
+    // so that no load is waited for before its round is complete.  The
+    // stencil block is read only by tiles with a flagged cell among their own
+    // and halo cells: elsewhere every flag sum is zero.
     const int cc = inb ? c : p.nc - 1;
     const int S = p.scnt[cc], C = inb ? (int)p.ccnt[cc] : 0;
     const int8_t own = p.smooth[cc];
@@ -796,16 +796,26 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const unsigned hid1 = (unsigned)p.halo[hb + min(tid + TILE, hmax)];
     const int8_t hs0 = p.smooth[hid0];
     const int8_t hs1 = p.smooth[hid1];
+    // A face's test counts only in a limited cell (the limiter condition, or
+    // a flagged extended stencil, known after the flag sums): the
+    // temperatures are requested here for cells meeting the condition and,
+    // after the sums, for the others that turn out limited.  (Requesting
+    // them in a flagged tile for every cell, with the stencil block, measured
+    // slower.)
     R tfo[PF], tfn[PF], tvd[PF];
-#pragma unroll
-    for (int fi = 0; fi < PF; ++fi) {
+    auto load_temps = [&](int fi) {
         const int sf = sfs[fi];
         const int f = (sf < 0 ? -sf : sf) - 1;
-        const bool in = fi < C && f < nf;
-        const int fs = in ? f : 0;
-        tfo[fi] = p.fvT[2 * fs];
-        tfn[fi] = p.fvT[2 * fs + 1];
-        tvd[fi] = p.tTVD[2 * fs + (sf < 0)];
+        if (fi < C && f < nf) {
+            tfo[fi] = p.fvT[2 * f];
+            tfn[fi] = p.fvT[2 * f + 1];
+            tvd[fi] = p.tTVD[2 * f + (sf < 0)];
+        }
+    };
+#pragma unroll
+    for (int fi = 0; fi < PF; ++fi) {
+        tfo[fi] = tfn[fi] = tvd[fi] = (R)0;
+        if (limited) load_temps(fi);
     }
     if (inb) sS[tid] = own;
     if (tid < hn) sS[TILE + tid] = hs0;
@@ -846,6 +856,10 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     // the condition lim < VALID_LIM_FACTOR taken from the reconstruction.
     int bvd = 0, bfaces = 0;
     const int isLimited = limited + (shockTypeExt > 0);
+    if (isLimited && !limited) {
+#pragma unroll
+        for (int fi = 0; fi < PF; ++fi) load_temps(fi);
+    }
     auto face = [&](int sf, R TfOwn, R TfNei, R Tf_cur_TVD) {
         const int isNei = sf < 0;
         const R Tf_cur = (R)isNei * TfNei + (R)(1 - isNei) * TfOwn;
@@ -861,7 +875,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
             const int f = (sf < 0 ? -sf : sf) - 1;
             if (f >= nf)
                 bfaces += 1;
-            else
+            else if (isLimited)
                 face(sf, tfo[fi], tfn[fi], tvd[fi]);
         }
     }
@@ -870,7 +884,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
         const int f = (sf < 0 ? -sf : sf) - 1;
         if (f >= nf)
             bfaces += 1;
-        else
+        else if (isLimited)
             face(sf, p.fvT[2 * f], p.fvT[2 * f + 1], p.tTVD[2 * f + (sf < 0)]);
     }
     if (bfaces >= 2) bvd = 1;  // corner cells use the second-order reconstruction
