@@ -1077,8 +1077,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R
                          face_limiter(rL.w, shL)};
         const v4 lR = v4{face_limiter(rR.x, shR), face_limiter(rR.y, shR), face_limiter(rR.z, shR),
                          face_limiter(rR.w, shR)};
-        const v2 gpL = p.gp[own], guL = p.gu[own], gvL = p.gv[own], gTL = p.gT[own];
-        const v2 gpR = p.gp[nei], guR = p.gu[nei], gvR = p.gv[nei], gTR = p.gT[nei];
+        // face-centre O3 states of both sides
+        v4 fL0{}, fR0{};
+        if constexpr (!GP) {
+            if constexpr (sizeof(R) == 8) {  // one 2-vector per variable
+                const v2 sp = reinterpret_cast<const v2*>(p.fvp)[f], su = reinterpret_cast<const v2*>(p.fvu)[f];
+                const v2 sv = reinterpret_cast<const v2*>(p.fvv)[f], sT = reinterpret_cast<const v2*>(p.fvT)[f];
+                fL0 = v4{sp.x, su.x, sv.x, sT.x};
+                fR0 = v4{sp.y, su.y, sv.y, sT.y};
+            } else {
+                fL0 = p.fv[2 * f];
+                fR0 = p.fv[2 * f + 1];
+            }
+        }
+        const v2 gpL = p.gp[own], gTL = p.gT[own];
+        const v2 gpR = p.gp[nei], gTR = p.gT[nei];
+        // The limited linear velocities enter the flux only on a TVD side: on
+        // an O3 side they are blended with weight 0, which adds a zero to the
+        // O3 velocity, so they change nothing unless that velocity is itself a
+        // zero (whose sign the added zero decides).  Their gradients are read
+        // only where they can matter (at the face centre).  A non-finite
+        // velocity gradient beside a finite O3 state (a degenerate
+        // least-squares stencil) would turn that blend into NaN in the
+        // reference; here it is not read.
+        v2 guL{}, gvL{}, guR{}, gvR{};
+        if (GP || isTVD_L || fL0.y == (R)0 || fL0.z == (R)0) {
+            guL = p.gu[own];
+            gvL = p.gv[own];
+        }
+        if (GP || isTVD_R || fR0.y == (R)0 || fR0.z == (R)0) {
+            guR = p.gu[nei];
+            gvR = p.gv[nei];
+        }
         if (DT) {
             const double d = cfl_face_dt(g0, g1, wL, wR, Rgas, Cv, CFL);
             dtf = d < dtf ? d : dtf;
@@ -1133,17 +1163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FluxWaves<R
         };
         if constexpr (!GP) {
             R rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux;
-            v4 fL, fR;
-            if constexpr (sizeof(R) == 8) {  // both sides' states: one 2-vector per variable
-                const v2 sp = reinterpret_cast<const v2*>(p.fvp)[f], su = reinterpret_cast<const v2*>(p.fvu)[f];
-                const v2 sv = reinterpret_cast<const v2*>(p.fvv)[f], sT = reinterpret_cast<const v2*>(p.fvT)[f];
-                fL = v4{sp.x, su.x, sv.x, sT.x};
-                fR = v4{sp.y, su.y, sv.y, sT.y};
-            } else {
-                fL = p.fv[2 * f];
-                fR = p.fv[2 * f + 1];
-            }
-            point_flux(dxL, dyL, dxR, dyR, fL, fR, rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
+            point_flux(dxL, dyL, dxR, dyR, fL0, fR0, rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux);
             if constexpr (sizeof(R) == 4)
                 p.flux32[f] = float4{rhoFlux, rhoUxFlux, rhoUyFlux, rhoEFlux};
             else
