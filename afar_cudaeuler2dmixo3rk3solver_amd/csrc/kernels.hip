@@ -761,7 +761,7 @@ __global__ __launch_bounds__(4 * TILE) AFAR_RECONQ_ATTR void k_reconstruct_q(Dev
 template <typename R, int TILE>
 __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     using v4 = typename VT<R>::v4;
-    constexpr int PF = 4;  // face slots whose data is requested before the barrier
+    constexpr int PF = 4;  // face slots whose data is requested in the first load rounds
     extern __shared__ __align__(16) unsigned char smem_raw[];
     int8_t* sS = reinterpret_cast<int8_t*>(smem_raw);
     uint16_t* sSt = reinterpret_cast<uint16_t*>(smem_raw + p.lds_bvd_sten_off);
@@ -770,7 +770,7 @@ __global__ __launch_bounds__(TILE) void k_bvd(DevPtrs<R> p) {
     const int tid = threadIdx.x, c = tile * TILE + tid;
     const int maxS = p.maxS, maxC = p.maxC, nf = p.nf;
     const bool inb = c < p.nc;
-    // Two rounds of independent loads before the barrier: (1) the cell's
+    // Two rounds of independent loads before the tile's flag vote: (1) the cell's
     // counts, flag and limiter condition, the first slots' faces and halo
     // ids; (2) halo flags and, in cells meeting the limiter condition, the
     // slots' face temperatures (both sides' O3 values and this side's limited
