@@ -114,18 +114,22 @@ def test_determinism_repeat(afar):
     assert np.array_equal(a.conserved(), b.conserved())
 
 
-def test_tiles_agree(afar):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tiles_agree(afar, precision):
     """The LDS tile size changes the launch shape, not the result: every tile
-    equals the oracle after 20 device-time-step steps of the corner (shock
-    cells present, so the BVD pass's flag vote takes both branches; tiles of
-    128 and 256 cells vote across several wavefronts)."""
-    say("case: tiles 64/128/256, corner 100x50, 20 steps, against the oracle")
-    mesh, W0, sc0 = _corner(afar)
-    orc = O.OracleState(mesh.reference_arrays(sc0, W0), sc0.R, sc0.Cv)
+    equals the oracle of its precision after 20 device-time-step steps of the
+    corner (shock cells present, so the BVD pass's flag vote takes both
+    branches; tiles of 128 and 256 cells vote across several wavefronts)."""
+    from afar_cudaeuler2dmixo3rk3solver_amd import cases
+    say("case: tiles 64/128/256, %s, corner 100x50, 20 steps, against the oracle" % precision)
+    mesh, W0 = cases.corner_case(nx=100, ny=50)
+    sc0 = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, precision=precision)
+    mesh.precompute(sc0)
+    orc = O.OracleState(mesh.reference_arrays(sc0, W0), sc0.R, sc0.Cv, mixed=precision == "mixed")
     orc.run(20, sc0.maxCo)
     assert (orc.smooth_flags != 0).any() and (orc.smooth_flags == 0).any()
     for tile in (64, 128, 256):
-        sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, tile=tile)
+        sc = afar.schemes(cOrder=7, fOrder=2, maxCo=0.5, tile=tile, precision=precision)
         s = afar.BasicCUDASolver(mesh, sc, W0)
         s.evolve_auto(20)
         assert np.array_equal(s.conserved()[mesh.permutation()], orc.U), tile
