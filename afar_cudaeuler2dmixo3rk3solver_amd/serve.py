@@ -104,7 +104,9 @@ def footprint_bytes(mesh, precision):
     slots = nslot * (2 * 1 + 4 + 16 + 16 + 2 * w) + (nc + 63) // 64 * 64 * maxS * 2
     rbf = nslot * (maxS + 1) * w
     faces = nf * (8 + 64 + 2 * 4 * w + 2 * w) + (nf + nb) * 32 + nb * (4 + 1 + 64)
-    centres = 16 * (nc + nb + nf + nb) if precision == "fp64" else 0  # cell and face centres (FP64 reconstruction)
+    # cell and face centres (the FP64 reconstruction, the face passes), the
+    # periodic shifts (counted always: an upper bound), Sf and magSf
+    centres = 16 * (nc + nb + nf + nb) + 16 * nf + 24 * nf
     return int(cell + slots + rbf + faces + centres + 2 * nc * 4 + (1 << 20))
 
 
