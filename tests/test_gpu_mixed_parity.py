@@ -179,3 +179,34 @@ def test_mixed_flux_points_parity(afar, forder, lanes):
     gpu.run_stage(2)
     assert np.array_equal(gpu.fetch("flux", np.float64, orc.a["Flux"].shape), orc.a["Flux"])
     gpu.close()
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_mixed_max_stencil_fan(afar, lanes):
+    """A fan of 20 triangles inside a slip wall: every stencil has the
+    largest size the reference accepts (20) and the mesh is one partial tile;
+    200 steps."""
+    from tests.test_gpu_parity import fan_case
+    say("case: mixed, fan of 20 triangles (stencils of 20), lanes %d, 200 steps" % lanes)
+    mesh, W0 = fan_case(afar)
+    gpu, orc, sc = make_pair(afar, mesh, W0, reconLanes=lanes)
+    assert mesh.sizes()["maxS"] == 20
+    step_both(gpu, orc, sc, 200)
+    assert_state_equal(gpu, orc, mesh)
+
+
+@pytest.mark.parametrize("lanes", [1, 4])
+@pytest.mark.parametrize("kind", ["one square", "two squares", "one triangle", "strip of three quads"])
+def test_mixed_tiny_meshes(afar, kind, lanes):
+    """The smallest meshes (no internal face, where the time step is the
+    1e5 cap; two cells; one triangle; three quads), 20 steps with host time
+    steps, then 20 device-time-step graph steps from the start."""
+    from tests.test_gpu_parity import tiny_case
+    mesh, W0 = tiny_case(afar, kind)
+    say("case: mixed, %s (nc=%d), lanes %d, 20 steps" % (kind, mesh.nc, lanes))
+    gpu, orc, sc = make_pair(afar, mesh, W0, reconLanes=lanes)
+    step_both(gpu, orc, sc, 20)
+    assert_state_equal(gpu, orc, mesh)
+    g = afar.BasicCUDASolver(mesh, sc, W0)
+    g.evolve_auto(20)
+    assert np.array_equal(g.conserved()[mesh.permutation()], orc.U)
